@@ -1,0 +1,204 @@
+/*
+ * turner_amd.h -- C ABI of the MI355X (gfx950) wavefront path tracer that
+ * replaces turner's pathtracer hot path: trace() (pathtracer.cpp:14-102)
+ * over KDTree / KDTreeIntersection::intersect (lib/kdtree.h:187-309,
+ * lib/kdtree.cpp:474-607).
+ *
+ * Plain C types only: host arrays are const float* / uint64_t*, device arrays
+ * are raw device pointers obtained from hipMalloc (or torch), streams are
+ * hipStream_t passed as void*.  Every entry point returns 0 (TN_OK) or a
+ * negative status and never throws; tn_last_error() gives a thread-local
+ * message for the last failure.  Scene handles are immutable after creation
+ * and may be shared by host threads.
+ */
+#ifndef TURNER_AMD_H
+#define TURNER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TN_OK 0
+#define TN_ERR_INVALID (-1)  /* bad handle or argument */
+#define TN_ERR_OOM (-2)      /* host or device allocation failed */
+#define TN_ERR_HIP (-3)      /* HIP runtime error (message in tn_last_error) */
+#define TN_ERR_QUEUE (-4)    /* a device ray queue would overflow its capacity */
+#define TN_ERR_NODEVICE (-5) /* no usable GPU */
+#define TN_ERR_LIMIT (-6)    /* size beyond a documented limit */
+#define TN_ERR_INCOMPLETE (-7) /* a traversal hit its iteration cap */
+
+/* Invalid triangle id, as OptionalId{} of lib/kdtree.h:156-181 (MAX_TRIANGLE_ID). */
+#define TN_INVALID_ID (1u << 30)
+/* Largest triangle count (ids are 30 bits, lib/kdtree.h:66). */
+#define TN_MAX_TRIANGLES ((1u << 30) - 1u)
+
+typedef struct tn_scene tn_scene;
+
+typedef struct {
+    uint32_t num_triangles; /* KDTree::num_triangles  lib/kdtree.h:219 */
+    uint32_t num_nodes;     /* KDTree::num_nodes (reference FlatNode count) lib/kdtree.h:218 */
+    uint32_t height;        /* KDTree::height  lib/kdtree.h:197-216 */
+    uint32_t device_nodes;  /* nodes of the device layout (one per inner node / leaf) */
+    uint32_t device_refs;   /* triangle references held by device leaves */
+    uint32_t num_emitters;  /* triangles with emission > 0 */
+    float box[6];           /* KDTree::box  lib/kdtree.h:221 (min xyz, max xyz) */
+    double build_ms;        /* host SAH build time */
+} tn_scene_info;
+
+/* Scene creation flags */
+#define TN_SCENE_SERIAL_BUILD 1u /* build the kd-tree on one host thread */
+
+/*
+ * Replaces KDTree::KDTree(Triangles) (lib/kdtree.h:195, lib/kdtree.cpp:474-490):
+ * precomputes the triangles (lib/triangle.h:22-39), builds the SAH kd-tree on
+ * the host with the reference's algorithm and constants (lib/kdtree.cpp:124-467)
+ * and uploads nodes, triangle references, triangle records and shading data
+ * to the device.  verts9: 9 floats per triangle (v0 v1 v2).  normals9: vertex
+ * normals (NULL: face normals).  albedo4: RGBA diffuse per triangle (NULL:
+ * 0.8 grey, alpha 1).  emission4: RGBA emission per triangle (NULL: none).
+ */
+int tn_scene_create(const float* verts9, const float* normals9, const float* albedo4,
+                    const float* emission4, uint32_t num_triangles, uint32_t flags,
+                    tn_scene** out);
+/* As tn_scene_create, but with a tree the caller already holds in the
+ * reference's FlatNode format (lib/kdtree.h:62-154) and its root box, e.g.
+ * from tn_tree_cache_load.  The tree is validated before upload. */
+int tn_scene_create_with_tree(const float* verts9, const float* normals9,
+                              const float* albedo4, const float* emission4,
+                              uint32_t num_triangles, const uint64_t* flat_nodes,
+                              size_t num_nodes, const float* box6, tn_scene** out);
+int tn_scene_destroy(tn_scene* scene);
+int tn_scene_get_info(const tn_scene* scene, tn_scene_info* info);
+/* Copies the reference-format FlatNode array (cap = capacity in nodes). */
+int tn_scene_get_flat_nodes(const tn_scene* scene, uint64_t* out, size_t cap);
+/* Copies the 16-float triangle records: v0 n u v uv vv uu denom. */
+int tn_scene_get_triangle_records(const tn_scene* scene, float* out16, size_t cap);
+
+/* Host-only (no GPU needed): the SAH build of tn_scene_create, returning the
+ * reference-format FlatNode array (free with tn_buffer_free), its root box
+ * and KDTree::height.  flags: TN_SCENE_SERIAL_BUILD. */
+int tn_kdtree_build(const float* verts9, uint32_t num_triangles, uint32_t flags,
+                    uint64_t** flat_nodes, size_t* num_nodes, float* box6, uint32_t* height);
+void tn_buffer_free(void* p);
+
+/* kd-tree cache replacing ./kdtree.cache (main.cpp:144-167).  Unlike the
+ * reference's cache, the file is versioned and keyed by a 64-bit hash of the
+ * triangle data: tn_tree_cache_load returns TN_ERR_INVALID for a cache that
+ * belongs to other geometry (or is truncated/corrupt), so a stale cache is
+ * rebuilt instead of silently rendering the wrong scene. */
+int tn_tree_cache_save(const char* path, const float* verts9, uint32_t num_triangles,
+                       const uint64_t* flat_nodes, size_t num_nodes, const float* box6);
+int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_triangles,
+                       uint64_t** flat_nodes, size_t* num_nodes, float* box6);
+
+/* Intersection flags */
+#define TN_ISECT_EXHAUSTIVE 1u /* visit every leaf the ray crosses (reference order) */
+#define TN_ISECT_MT 2u         /* FP32 Moeller-Trumbore leaf test instead of the reference's */
+#define TN_ISECT_COUNT 4u      /* accumulate work counters (tn_get_counters) */
+
+/*
+ * Batched twin of KDTreeIntersection::intersect(ray, r, a, b) (lib/kdtree.h:291,
+ * lib/kdtree.cpp:515-607).  Device pointers: o3, d3 hold n rays as xyz
+ * triples; outputs id (TN_INVALID_ID on miss), r (ray parameter of the hit,
+ * FLT_MAX on miss), s, t (barycentrics of v1, v2; 0 on miss).  Closest hit
+ * with the reference's strict-< tie rule in near-first visit order.
+ * stream: hipStream_t or NULL for the null stream.  Asynchronous.
+ */
+int tn_intersect(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
+                 uint32_t* id, float* r, float* s, float* t, uint32_t flags, void* stream);
+/* Host-pointer variant (copies in and out, synchronous). */
+int tn_intersect_host(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
+                      uint32_t* id, float* r, float* s, float* t, uint32_t flags);
+/* Shadow / visibility query with the reference's predicate
+ * (pathtracer.cpp:49-53): occluded[i] = 1 iff some triangle is hit at
+ * 0 <= r < tmax[i].  Device pointers.  Asynchronous. */
+int tn_occluded(tn_scene* scene, const float* o3, const float* d3, const float* tmax,
+                uint32_t n, uint8_t* occluded, uint32_t flags, void* stream);
+/* Work counters of the launches made with TN_ISECT_COUNT since the last reset:
+ * [0] inner nodes visited, [1] leaves visited, [2] triangle references read,
+ * [3] ray-triangle tests, [4] rays, [5] stack restarts. reset != 0 zeroes them. */
+int tn_get_counters(tn_scene* scene, uint64_t* out6, int reset);
+
+/* Camera as lib/types.h:89-144: ray direction =
+ * trafo * (-delta_x (1 - 2 px / w), delta_y (1 - 2 py / h), -1), origin pos. */
+typedef struct {
+    float pos[3];
+    float trafo[9]; /* row-major 3x3 */
+    float delta_x, delta_y;
+} tn_camera;
+
+/* Point light as lib/types.h:81-84 */
+typedef struct {
+    float pos[3];
+    float color[4];
+} tn_light;
+
+/* Render flags */
+#define TN_RENDER_COSINE 1u      /* cosine-weighted hemisphere (default: uniform, as the reference) */
+#define TN_RENDER_AREA_LIGHTS 2u /* emissive triangles: NEE + camera-visible emission */
+#define TN_RENDER_REF_JITTER 4u  /* pixel jitter of main.cpp:201-206 (xorshift64* seeded 42 per row) */
+#define TN_RENDER_GUIDED 8u      /* learned guiding proposal (needs a tn_guide) */
+#define TN_RENDER_EXHAUSTIVE 16u /* reference traversal order without early exit */
+
+typedef struct {
+    uint32_t width, height;
+    int32_t max_depth;     /* --max-depth */
+    int32_t mc_samples;    /* --monte-carlo-samples */
+    int32_t pixel_samples; /* --pixel-samples */
+    uint32_t flags;        /* TN_RENDER_* */
+    uint64_t seed;
+    float bg[4];           /* --background (alpha 1) */
+    float exposure;        /* --exposure */
+    float inverse_gamma;   /* --inverse-gamma */
+    int32_t gamma_enabled; /* !--no-gamma-correction */
+    uint32_t chunk_paths;  /* primary samples per wavefront chunk (0: automatic) */
+    uint32_t sample_offset;/* first pixel-sample index (progressive rendering) */
+    uint32_t reserved;
+} tn_render_params;
+
+typedef struct {
+    uint64_t rays;        /* closest-hit segments, the reference's num_rays (pathtracer.cpp:21) */
+    uint64_t primary_rays;/* num_prim_rays (main.cpp:211) */
+    uint64_t shadow_rays; /* visibility queries */
+    double ms;            /* wall time of the render loop (device-synchronised) */
+} tn_render_stats;
+
+/*
+ * Replaces the pixel loop of main.cpp:187-226 calling trace() per pixel
+ * sample.  linear_rgba (host, w*h*4, may be NULL): mean radiance over the
+ * pixel samples before exposure, as image(x,y) after main.cpp:216.
+ * display_rgba (host, may be NULL): after exposure and gamma (main.cpp:218-223).
+ */
+int tn_render(tn_scene* scene, const tn_camera* cam, const tn_light* lights,
+              uint32_t num_lights, const tn_render_params* params, float* linear_rgba,
+              float* display_rgba, tn_render_stats* stats);
+
+/* Same as tn_render but leaves the result in device memory: accum (device,
+ * w*h*4 floats) receives the SUM over pixel samples (not divided) and is
+ * accumulated into, not overwritten. */
+int tn_render_accumulate(tn_scene* scene, const tn_camera* cam, const tn_light* lights,
+                         uint32_t num_lights, const tn_render_params* params, float* accum,
+                         tn_render_stats* stats, void* stream);
+
+/* Tone map (lib/effects.h:15-49) of device RGBA sums: out = gamma(exposure(sum/spp)). */
+int tn_tonemap(const float* accum, float* out, uint32_t num_pixels, float inv_spp,
+               float exposure, float inverse_gamma, int gamma_enabled, void* stream);
+
+/* PPM "P3" text of an RGBA image as lib/raster.h:79-100 writes it
+ * (int(clamp(255 * c * a)) per channel).  Returns the byte count written
+ * (excluding the terminating NUL) or a negative status; buf may be NULL to
+ * query the size. */
+long tn_write_ppm(const float* rgba, uint32_t width, uint32_t height, char* buf, size_t cap);
+
+const char* tn_last_error(void);
+/* Device name and compute-unit count of the current HIP device. */
+int tn_device_info(char* name, size_t cap, int* compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C ABI)")
+    config.addinivalue_line("markers", "slow: long-running (full-size workloads)")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Skips nothing: GPU tests must fail loudly when the HIP path is unusable."""
+    import turner_amd
+    name, cus = turner_amd.device_info()
+    print(f"[gpu] {name}, {cus} CUs", flush=True)
+    return name

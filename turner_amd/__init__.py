@@ -1,0 +1,276 @@
+"""turner_amd: MI355X (gfx950) wavefront path tracer for turner's pathtracer hot path.
+
+Python mirror of the reference's host interface over the C ABI of
+libturner_amd.so (include/turner_amd.h):
+
+* KDTree(verts, ...)                -- KDTree::KDTree(Triangles)   lib/kdtree.h:195
+    .height() .num_nodes() .num_triangles() .box()                  lib/kdtree.h:197-221
+* KDTreeIntersection(tree)          -- lib/kdtree.h:270-309
+    .intersect(o, d) -> (id | None, r, s, t)                        lib/kdtree.cpp:515-578
+    .intersect_batch(o, d)          -- batched twin (device tensors or host arrays)
+* TracerConfig                      -- config.h:103-153 (+ pathtracer.h:4-25 flags)
+* render(tree, camera, lights, conf) -- the main.cpp:187-226 pixel loop over trace()
+
+Every call runs the HIP kernels; there is no CPU fallback.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import TN_INVALID_ID, TurnerError, check, lib, ptr
+
+__all__ = ["KDTree", "KDTreeIntersection", "TracerConfig", "render", "TurnerError",
+           "TN_INVALID_ID", "device_info", "build_tree", "tree_cache_save", "tree_cache_load",
+           "write_ppm"]
+
+
+def build_tree(verts, serial=False):
+    """Host-only SAH build (no GPU): (reference FlatNode array, box, height)."""
+    v = _f32(verts, 9)
+    nodes = C.POINTER(C.c_uint64)()
+    nn = C.c_size_t(0)
+    box = np.zeros(6, np.float32)
+    h = C.c_uint32(0)
+    check(lib().tn_kdtree_build(ptr(v), len(v), _lib.TN_SCENE_SERIAL_BUILD if serial else 0,
+                                C.byref(nodes), C.byref(nn), ptr(box), C.byref(h)))
+    out = np.ctypeslib.as_array(nodes, shape=(nn.value,)).copy()
+    lib().tn_buffer_free(nodes)
+    return out, box, h.value
+
+
+def tree_cache_save(path, verts, nodes, box):
+    v = _f32(verts, 9)
+    nodes = np.ascontiguousarray(nodes, np.uint64)
+    box = np.ascontiguousarray(box, np.float32)
+    check(lib().tn_tree_cache_save(path.encode(), ptr(v), len(v), ptr(nodes), len(nodes),
+                                   ptr(box)))
+
+
+def tree_cache_load(path, verts):
+    """Returns (nodes, box) or raises TurnerError for a missing/stale/corrupt cache."""
+    v = _f32(verts, 9)
+    nodes = C.POINTER(C.c_uint64)()
+    nn = C.c_size_t(0)
+    box = np.zeros(6, np.float32)
+    check(lib().tn_tree_cache_load(path.encode(), ptr(v), len(v), C.byref(nodes), C.byref(nn),
+                                   ptr(box)))
+    out = np.ctypeslib.as_array(nodes, shape=(nn.value,)).copy()
+    lib().tn_buffer_free(nodes)
+    return out, box
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cus = C.c_int(0)
+    check(lib().tn_device_info(name, 256, C.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+def _f32(a, cols):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(-1, cols)
+
+
+class KDTree:
+    """Scene handle: triangles + SAH kd-tree resident in HBM.
+
+    verts (N, 9) float32; normals (N, 9) or None (face normals); albedo (N, 4)
+    or None (0.8 grey); emission (N, 4) or None.  `flat_nodes`/`box` upload a
+    tree built earlier (e.g. from the tree cache) instead of building one.
+    """
+
+    def __init__(self, verts, normals=None, albedo=None, emission=None, serial=False,
+                 flat_nodes=None, box=None):
+        self._v = _f32(verts, 9)
+        self._n = _f32(normals, 9)
+        self._a = _f32(albedo, 4)
+        self._e = _f32(emission, 4)
+        n = len(self._v)
+        h = C.c_void_p()
+        if flat_nodes is not None:
+            fl = np.ascontiguousarray(flat_nodes, dtype=np.uint64)
+            bx = np.ascontiguousarray(box, dtype=np.float32)
+            check(lib().tn_scene_create_with_tree(ptr(self._v), ptr(self._n), ptr(self._a),
+                                                  ptr(self._e), n, ptr(fl), len(fl), ptr(bx),
+                                                  C.byref(h)))
+        else:
+            flags = _lib.TN_SCENE_SERIAL_BUILD if serial else 0
+            check(lib().tn_scene_create(ptr(self._v), ptr(self._n), ptr(self._a), ptr(self._e),
+                                        n, flags, C.byref(h)))
+        self.handle = h
+        self._info = _lib.SceneInfo()
+        check(lib().tn_scene_get_info(self.handle, C.byref(self._info)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tn_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # lib/kdtree.h:197-224
+    def height(self):
+        return self._info.height
+
+    def num_nodes(self):
+        return self._info.num_nodes
+
+    def num_triangles(self):
+        return self._info.num_triangles
+
+    def box(self):
+        return np.array(self._info.box, np.float32)
+
+    @property
+    def info(self):
+        return self._info
+
+    def flat_nodes(self):
+        out = np.zeros(self._info.num_nodes, np.uint64)
+        check(lib().tn_scene_get_flat_nodes(self.handle, ptr(out), len(out)))
+        return out
+
+    def triangle_records(self):
+        out = np.zeros((self._info.num_triangles, 16), np.float32)
+        check(lib().tn_scene_get_triangle_records(self.handle, ptr(out), len(out)))
+        return out
+
+    @property
+    def verts(self):
+        return self._v
+
+
+class KDTreeIntersection:
+    """lib/kdtree.h:270-309.  intersect() for one ray mirrors the reference's
+    out-parameter call; intersect_batch() is the batched GPU twin."""
+
+    def __init__(self, tree, flags=0):
+        self.tree = tree
+        self.flags = flags
+
+    def intersect(self, o, d):
+        ids, r, s, t = self.intersect_batch(np.asarray([o], np.float32),
+                                            np.asarray([d], np.float32))
+        if ids[0] == TN_INVALID_ID:
+            return None, float(r[0]), 0.0, 0.0
+        return int(ids[0]), float(r[0]), float(s[0]), float(t[0])
+
+    def intersect_batch(self, o, d, flags=None):
+        """o, d: (n, 3) float32 host arrays or device tensors.  Returns
+        (id, r, s, t) of the same kind."""
+        flags = self.flags if flags is None else flags
+        if hasattr(o, "is_cuda") and o.is_cuda:
+            import torch
+            n = o.shape[0]
+            ids = torch.empty(n, dtype=torch.int32, device=o.device)
+            r = torch.empty(n, dtype=torch.float32, device=o.device)
+            s = torch.empty_like(r)
+            t = torch.empty_like(r)
+            stream = torch.cuda.current_stream(o.device).cuda_stream
+            check(lib().tn_intersect(self.tree.handle, ptr(o), ptr(d), n, ptr(ids), ptr(r),
+                                     ptr(s), ptr(t), flags, stream))
+            return ids, r, s, t
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        n = len(o)
+        ids = np.zeros(n, np.uint32)
+        r = np.zeros(n, np.float32)
+        s = np.zeros(n, np.float32)
+        t = np.zeros(n, np.float32)
+        check(lib().tn_intersect_host(self.tree.handle, ptr(o), ptr(d), n, ptr(ids), ptr(r),
+                                      ptr(s), ptr(t), flags))
+        return ids, r, s, t
+
+
+@dataclass
+class TracerConfig:
+    """config.h:45-60,103-153 with the pathtracer USAGE defaults (pathtracer.h:4-25)."""
+    width: int = 640
+    aspect: float = 1.0
+    bg_color: tuple = (0.0, 0.0, 0.0, 1.0)
+    num_threads: int = 1
+    inverse_gamma: float = 0.454545
+    exposure: float = 1.0
+    gamma_correction_enabled: bool = True
+    max_recursion_depth: int = 3
+    num_pixel_samples: int = 1
+    num_monte_carlo_samples: int = 8
+    # extensions (no reference counterpart)
+    flags: int = 0
+    seed: int = 1
+    chunk_paths: int = 0
+
+    def height(self):
+        return int(self.width / self.aspect)
+
+    def check(self):  # config.h:29-33,119-126
+        assert 0 < self.aspect and 1 <= self.num_threads and 0 <= self.exposure
+        assert 0 < self.max_recursion_depth and 1 <= self.num_pixel_samples
+        assert 0 <= self.num_monte_carlo_samples
+
+
+def _camera(cam):
+    c = _lib.Camera()
+    c.pos[:] = [float(x) for x in np.asarray(cam["pos"]).ravel()]
+    c.trafo[:] = [float(x) for x in np.asarray(cam["trafo"]).ravel()]
+    c.delta_x = float(cam["delta_x"])
+    c.delta_y = float(cam["delta_y"])
+    return c
+
+
+def _lights(lights):
+    arr = (_lib.Light * max(1, len(lights)))()
+    for k, (p, col) in enumerate(lights):
+        arr[k].pos[:] = [float(x) for x in p]
+        arr[k].color[:] = [float(x) for x in col]
+    return arr
+
+
+def _params(conf, width, height):
+    p = _lib.RenderParams()
+    p.width, p.height = width, height
+    p.max_depth = conf.max_recursion_depth
+    p.mc_samples = conf.num_monte_carlo_samples
+    p.pixel_samples = conf.num_pixel_samples
+    p.flags = conf.flags
+    p.seed = conf.seed
+    p.bg[:] = [float(x) for x in conf.bg_color]
+    p.exposure = conf.exposure
+    p.inverse_gamma = conf.inverse_gamma
+    p.gamma_enabled = 1 if conf.gamma_correction_enabled else 0
+    p.chunk_paths = conf.chunk_paths
+    return p
+
+
+def render(tree, cam, lights, conf, display=False):
+    """Renders on the GPU.  Returns (linear RGBA (h, w, 4) mean over pixel
+    samples before exposure, stats dict[, display RGBA after exposure/gamma])."""
+    conf.check()
+    w, h = conf.width, conf.height()
+    lin = np.zeros((h, w, 4), np.float32)
+    disp = np.zeros((h, w, 4), np.float32) if display else None
+    st = _lib.RenderStats()
+    ls = _lights(lights)
+    check(lib().tn_render(tree.handle, C.byref(_camera(cam)), ls, len(lights),
+                          C.byref(_params(conf, w, h)), ptr(lin), ptr(disp), C.byref(st)))
+    stats = {"rays": st.rays, "primary_rays": st.primary_rays, "shadow_rays": st.shadow_rays,
+             "ms": st.ms}
+    return (lin, stats, disp) if display else (lin, stats)
+
+
+def write_ppm(rgba):
+    """PPM P3 text of an (h, w, 4) image (lib/raster.h:79-100)."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    h, w = rgba.shape[:2]
+    n = check(lib().tn_write_ppm(ptr(rgba), w, h, None, 0))
+    buf = C.create_string_buffer(n + 1)
+    check(lib().tn_write_ppm(ptr(rgba), w, h, buf, n + 1))
+    return buf.value.decode()

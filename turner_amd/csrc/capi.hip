@@ -1,0 +1,483 @@
+// capi.hip -- scene management, batched intersection and visibility entry
+// points of libturner_amd (C ABI declared in include/turner_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "tn_scene.h"
+
+namespace tn {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory)
+        return set_error(TN_ERR_OOM, "%s: %s", what, hipGetErrorString(e));
+    return set_error(TN_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+namespace {
+
+// Batched closest hit: one ray per lane, LDS short stack of kStack entries.
+template <bool EXH, bool MT, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_intersect(SceneView sc, const float* __restrict__ o3,
+                                                      const float* __restrict__ d3, uint32_t n,
+                                                      uint32_t* __restrict__ out_id,
+                                                      float* __restrict__ out_r,
+                                                      float* __restrict__ out_s,
+                                                      float* __restrict__ out_t,
+                                                      unsigned long long* counters,
+                                                      uint32_t* status) {
+    __shared__ uint2 stk[kStack * kBlock];
+    __shared__ uint32_t red[6];
+    if (COUNT && threadIdx.x < 6) red[threadIdx.x] = 0;
+    if (COUNT) __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    TraversalCounters cnt;
+    if (i < n) {
+        const float3 o = make_float3(o3[3 * (size_t)i], o3[3 * (size_t)i + 1], o3[3 * (size_t)i + 2]);
+        const float3 d = make_float3(d3[3 * (size_t)i], d3[3 * (size_t)i + 1], d3[3 * (size_t)i + 2]);
+        float r, s, t;
+        const uint32_t id = traverse<kStack, EXH, MT, false, COUNT>(sc, o, d, 0.f, r, s, t,
+                                                                    stk + threadIdx.x, kBlock, cnt);
+        const bool hit = id != kInvalidId;
+        out_id[i] = id;
+        out_r[i] = hit ? r : FLT_MAX;
+        out_s[i] = hit ? s : 0.f;
+        out_t[i] = hit ? t : 0.f;
+        if (cnt.incomplete) atomicAdd(status, 1u);
+    }
+    if (COUNT) {
+        atomicAdd(&red[0], cnt.inner);
+        atomicAdd(&red[1], cnt.leaves);
+        atomicAdd(&red[2], cnt.refs);
+        atomicAdd(&red[3], cnt.tests);
+        atomicAdd(&red[4], i < n ? 1u : 0u);
+        atomicAdd(&red[5], cnt.restarts);
+        __syncthreads();
+        if (threadIdx.x < 6) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+    }
+}
+
+template <bool EXH, bool MT>
+__global__ __launch_bounds__(kBlock) void k_occluded(SceneView sc, const float* __restrict__ o3,
+                                                     const float* __restrict__ d3,
+                                                     const float* __restrict__ tmax, uint32_t n,
+                                                     uint8_t* __restrict__ occ, uint32_t* status) {
+    __shared__ uint2 stk[kStack * kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float3 o = make_float3(o3[3 * (size_t)i], o3[3 * (size_t)i + 1], o3[3 * (size_t)i + 2]);
+    const float3 d = make_float3(d3[3 * (size_t)i], d3[3 * (size_t)i + 1], d3[3 * (size_t)i + 2]);
+    TraversalCounters cnt;
+    float r, s, t;
+    const uint32_t id = traverse<kStack, EXH, MT, true, false>(sc, o, d, tmax[i], r, s, t,
+                                                               stk + threadIdx.x, kBlock, cnt);
+    occ[i] = id != kInvalidId ? 1 : 0;
+    if (cnt.incomplete) atomicAdd(status, 1u);
+}
+
+__global__ void k_tonemap(const float4* __restrict__ acc, float4* __restrict__ out, uint32_t n,
+                          float inv_spp, float exposure, float inv_gamma, int gamma_on) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = acc[i];
+    c.x *= inv_spp;
+    c.y *= inv_spp;
+    c.z *= inv_spp;
+    c.w *= inv_spp;
+    // lib/effects.h:15-49: exposure then gamma on RGB, alpha untouched
+    c.x = 1 - expf(-c.x * exposure);
+    c.y = 1 - expf(-c.y * exposure);
+    c.z = 1 - expf(-c.z * exposure);
+    if (gamma_on) {
+        c.x = powf(c.x, inv_gamma);
+        c.y = powf(c.y, inv_gamma);
+        c.z = powf(c.z, inv_gamma);
+    }
+    out[i] = c;
+}
+
+int check_device(int* dev) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return set_error(TN_ERR_NODEVICE, "no HIP device available (%s)",
+                         e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+    TN_HIP(hipGetDevice(dev));
+    return TN_OK;
+}
+
+template <class T>
+int dev_upload(T** dst, const T* src, size_t count) {
+    if (count == 0) count = 1;
+    TN_HIP(hipMalloc((void**)dst, count * sizeof(T)));
+    if (src) TN_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return TN_OK;
+}
+
+void free_device(tn_scene* sc) {
+    (void)hipFree(sc->d_nodes);
+    (void)hipFree(sc->d_refs);
+    (void)hipFree(sc->d_tris);
+    (void)hipFree(sc->d_normals);
+    (void)hipFree(sc->d_albedo);
+    (void)hipFree(sc->d_emission);
+    (void)hipFree(sc->d_emit_ids);
+    (void)hipFree(sc->d_emit_cdf);
+    (void)hipFree(sc->d_counters);
+    (void)hipFree(sc->d_status);
+}
+
+int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const float* albedo4,
+                 const float* emission4) {
+    const uint32_t n = sc->ntris;
+    std::string err;
+    if (!flat_to_device(sc->tree.flat, n, sc->tree.nodes, sc->tree.refs, err))
+        return set_error(TN_ERR_INVALID, "invalid kd-tree: %s", err.c_str());
+    sc->tree.height = tree_height(sc->tree.flat);
+    int rc;
+    if ((rc = dev_upload(&sc->d_nodes, reinterpret_cast<const uint2*>(sc->tree.nodes.data()),
+                         sc->tree.nodes.size())))
+        return rc;
+    if ((rc = dev_upload(&sc->d_refs, sc->tree.refs.data(), sc->tree.refs.size()))) return rc;
+    // device record: (v0, uv) (n, vv) (u, uu) (v, denom), one 64-byte line per triangle
+    std::vector<float4> rec((size_t)n * 4);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = &sc->pre16[16 * (size_t)i];
+        rec[4 * (size_t)i] = make_float4(p[0], p[1], p[2], p[12]);
+        rec[4 * (size_t)i + 1] = make_float4(p[3], p[4], p[5], p[13]);
+        rec[4 * (size_t)i + 2] = make_float4(p[6], p[7], p[8], p[14]);
+        rec[4 * (size_t)i + 3] = make_float4(p[9], p[10], p[11], p[15]);
+    }
+    if ((rc = dev_upload(&sc->d_tris, rec.data(), rec.size()))) return rc;
+    std::vector<float> nrm((size_t)n * 9);
+    if (normals9) {
+        std::memcpy(nrm.data(), normals9, nrm.size() * sizeof(float));
+    } else {
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) nrm[9 * (size_t)i + 3 * k + c] = sc->pre16[16 * (size_t)i + 3 + c];
+    }
+    if ((rc = dev_upload(&sc->d_normals, nrm.data(), nrm.size()))) return rc;
+    std::vector<float4> alb(n), emi(n);
+    std::vector<uint32_t> eids;
+    std::vector<float> ecdf;
+    double acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        alb[i] = albedo4 ? make_float4(albedo4[4 * (size_t)i], albedo4[4 * (size_t)i + 1],
+                                       albedo4[4 * (size_t)i + 2], albedo4[4 * (size_t)i + 3])
+                         : make_float4(0.8f, 0.8f, 0.8f, 1.f);
+        emi[i] = emission4 ? make_float4(emission4[4 * (size_t)i], emission4[4 * (size_t)i + 1],
+                                         emission4[4 * (size_t)i + 2], emission4[4 * (size_t)i + 3])
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (emi[i].x > 0 || emi[i].y > 0 || emi[i].z > 0) {
+            const float* p = &sc->pre16[16 * (size_t)i];
+            double ux = p[6], uy = p[7], uz = p[8], vx = p[9], vy = p[10], vz = p[11];
+            double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            acc += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            eids.push_back(i);
+            ecdf.push_back((float)acc);
+        }
+    }
+    sc->nemit = (uint32_t)eids.size();
+    sc->emit_area = (float)acc;
+    if ((rc = dev_upload(&sc->d_albedo, alb.data(), alb.size()))) return rc;
+    if ((rc = dev_upload(&sc->d_emission, emi.data(), emi.size()))) return rc;
+    if ((rc = dev_upload(&sc->d_emit_ids, eids.data(), eids.size()))) return rc;
+    if ((rc = dev_upload(&sc->d_emit_cdf, ecdf.data(), ecdf.size()))) return rc;
+    if ((rc = dev_upload<unsigned long long>(&sc->d_counters, nullptr, 8))) return rc;
+    if ((rc = dev_upload<uint32_t>(&sc->d_status, nullptr, 4))) return rc;
+    TN_HIP(hipMemset(sc->d_counters, 0, 8 * sizeof(unsigned long long)));
+    TN_HIP(hipMemset(sc->d_status, 0, 4 * sizeof(uint32_t)));
+    return TN_OK;
+}
+
+int prepare_triangles(tn_scene* sc, const float* verts9, uint32_t n) {
+    sc->ntris = n;
+    sc->pre16.resize((size_t)n * 16);
+    precompute_triangles(verts9, n, sc->pre16.data());
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = &sc->pre16[16 * (size_t)i];
+        for (int k = 0; k < 16; ++k)
+            if (!std::isfinite(p[k]))
+                return set_error(TN_ERR_INVALID,
+                                 "triangle %u is degenerate or not finite (the reference asserts)", i);
+    }
+    return TN_OK;
+}
+
+}  // namespace
+
+int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                     float* r, float* s, float* t, uint32_t flags, hipStream_t stream) {
+    if (n == 0) return TN_OK;
+    const SceneView v = sc->view();
+    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
+               cnt = flags & TN_ISECT_COUNT;
+    auto go = [&](auto kern) {
+        kern<<<grid, block, 0, stream>>>(v, o3, d3, n, id, r, s, t, sc->d_counters, sc->d_status);
+    };
+    if (cnt) {
+        if (exh) {
+            if (mt) go(k_intersect<true, true, true>);
+            else go(k_intersect<true, false, true>);
+        } else {
+            if (mt) go(k_intersect<false, true, true>);
+            else go(k_intersect<false, false, true>);
+        }
+    } else {
+        if (exh) {
+            if (mt) go(k_intersect<true, true, false>);
+            else go(k_intersect<true, false, false>);
+        } else {
+            if (mt) go(k_intersect<false, true, false>);
+            else go(k_intersect<false, false, false>);
+        }
+    }
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" {
+
+const char* tn_last_error(void) { return g_error.c_str(); }
+
+int tn_device_info(char* name, size_t cap, int* cus) {
+    int dev;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    hipDeviceProp_t p;
+    TN_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && cap) {
+        std::snprintf(name, cap, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cus) *cus = p.multiProcessorCount;
+    return TN_OK;
+}
+
+int tn_scene_create(const float* verts9, const float* normals9, const float* albedo4,
+                    const float* emission4, uint32_t n, uint32_t flags, tn_scene** out) {
+    if (!out) return set_error(TN_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!verts9 || n == 0) return set_error(TN_ERR_INVALID, "no triangles");
+    if (n > TN_MAX_TRIANGLES) return set_error(TN_ERR_LIMIT, "%u triangles exceed 2^30-1", n);
+    std::unique_ptr<tn_scene> sc(new (std::nothrow) tn_scene);
+    if (!sc) return set_error(TN_ERR_OOM, "host allocation failed");
+    int rc = check_device(&sc->device);
+    if (rc) return rc;
+    try {
+        if ((rc = prepare_triangles(sc.get(), verts9, n))) return rc;
+        auto t0 = std::chrono::steady_clock::now();
+        build_kdtree(verts9, n, !(flags & TN_SCENE_SERIAL_BUILD), sc->tree.flat, sc->tree.box);
+        sc->tree.build_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if ((rc = finish_scene(sc.get(), verts9, normals9, albedo4, emission4))) {
+            free_device(sc.get());
+            return rc;
+        }
+    } catch (const std::bad_alloc&) {
+        free_device(sc.get());
+        return set_error(TN_ERR_OOM, "host allocation failed during build");
+    }
+    *out = sc.release();
+    return TN_OK;
+}
+
+int tn_scene_create_with_tree(const float* verts9, const float* normals9, const float* albedo4,
+                              const float* emission4, uint32_t n, const uint64_t* flat,
+                              size_t num_nodes, const float* box6, tn_scene** out) {
+    if (!out) return set_error(TN_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!verts9 || n == 0 || !flat || num_nodes == 0 || !box6)
+        return set_error(TN_ERR_INVALID, "missing triangles or tree");
+    if (n > TN_MAX_TRIANGLES) return set_error(TN_ERR_LIMIT, "%u triangles exceed 2^30-1", n);
+    if (num_nodes >= (1u << 30)) return set_error(TN_ERR_LIMIT, "too many nodes");
+    std::unique_ptr<tn_scene> sc(new (std::nothrow) tn_scene);
+    if (!sc) return set_error(TN_ERR_OOM, "host allocation failed");
+    int rc = check_device(&sc->device);
+    if (rc) return rc;
+    try {
+        if ((rc = prepare_triangles(sc.get(), verts9, n))) return rc;
+        sc->tree.flat.assign(flat, flat + num_nodes);
+        std::memcpy(sc->tree.box, box6, 6 * sizeof(float));
+        if ((rc = finish_scene(sc.get(), verts9, normals9, albedo4, emission4))) {
+            free_device(sc.get());
+            return rc;
+        }
+    } catch (const std::bad_alloc&) {
+        free_device(sc.get());
+        return set_error(TN_ERR_OOM, "host allocation failed");
+    }
+    *out = sc.release();
+    return TN_OK;
+}
+
+int tn_scene_destroy(tn_scene* sc) {
+    if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
+    free_device(sc);
+    delete sc;
+    return TN_OK;
+}
+
+int tn_scene_get_info(const tn_scene* sc, tn_scene_info* info) {
+    if (!sc || !info) return set_error(TN_ERR_INVALID, "NULL argument");
+    info->num_triangles = sc->ntris;
+    info->num_nodes = (uint32_t)sc->tree.flat.size();
+    info->height = sc->tree.height;
+    info->device_nodes = (uint32_t)sc->tree.nodes.size();
+    info->device_refs = (uint32_t)sc->tree.refs.size();
+    info->num_emitters = sc->nemit;
+    std::memcpy(info->box, sc->tree.box, sizeof(info->box));
+    info->build_ms = sc->tree.build_ms;
+    return TN_OK;
+}
+
+int tn_scene_get_flat_nodes(const tn_scene* sc, uint64_t* out, size_t cap) {
+    if (!sc || !out) return set_error(TN_ERR_INVALID, "NULL argument");
+    if (cap < sc->tree.flat.size()) return set_error(TN_ERR_LIMIT, "capacity too small");
+    std::memcpy(out, sc->tree.flat.data(), sc->tree.flat.size() * sizeof(uint64_t));
+    return TN_OK;
+}
+
+int tn_scene_get_triangle_records(const tn_scene* sc, float* out16, size_t cap) {
+    if (!sc || !out16) return set_error(TN_ERR_INVALID, "NULL argument");
+    if (cap < sc->ntris) return set_error(TN_ERR_LIMIT, "capacity too small");
+    std::memcpy(out16, sc->pre16.data(), sc->pre16.size() * sizeof(float));
+    return TN_OK;
+}
+
+int tn_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                 float* r, float* s, float* t, uint32_t flags, void* stream) {
+    if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
+    if (n && (!o3 || !d3 || !id || !r || !s || !t)) return set_error(TN_ERR_INVALID, "NULL array");
+    return launch_intersect(sc, o3, d3, n, id, r, s, t, flags, (hipStream_t)stream);
+}
+
+int tn_intersect_host(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                      float* r, float* s, float* t, uint32_t flags) {
+    if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
+    if (n == 0) return TN_OK;
+    if (!o3 || !d3 || !id || !r || !s || !t) return set_error(TN_ERR_INVALID, "NULL array");
+    TN_HIP(hipSetDevice(sc->device));
+    char* buf = nullptr;
+    const size_t ray_bytes = (size_t)n * 12;
+    TN_HIP(hipMalloc(&buf, 2 * ray_bytes + (size_t)n * 16));
+    float* d_o = (float*)buf;
+    float* d_d = (float*)(buf + ray_bytes);
+    uint32_t* d_id = (uint32_t*)(buf + 2 * ray_bytes);
+    float* d_r = (float*)(d_id + n);
+    float* d_s = d_r + n;
+    float* d_t = d_s + n;
+    int rc = TN_OK;
+    hipError_t e;
+    if ((e = hipMemcpy(d_o, o3, ray_bytes, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d_d, d3, ray_bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = hip_error(e, "hipMemcpy(rays)");
+    } else if ((rc = launch_intersect(sc, d_o, d_d, n, d_id, d_r, d_s, d_t, flags, nullptr)) == TN_OK) {
+        if ((e = hipMemcpy(id, d_id, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(r, d_r, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(s, d_s, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = hip_error(e, "hipMemcpy(hits)");
+    }
+    (void)hipFree(buf);
+    if (rc == TN_OK) {
+        uint32_t st = 0;
+        TN_HIP(hipMemcpy(&st, sc->d_status, 4, hipMemcpyDeviceToHost));
+        if (st) {
+            TN_HIP(hipMemset(sc->d_status, 0, 4));
+            return set_error(TN_ERR_INCOMPLETE, "%u traversals hit the restart cap", st);
+        }
+    }
+    return rc;
+}
+
+int tn_occluded(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
+                uint8_t* occ, uint32_t flags, void* stream) {
+    if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
+    if (n == 0) return TN_OK;
+    if (!o3 || !d3 || !tmax || !occ) return set_error(TN_ERR_INVALID, "NULL array");
+    const SceneView v = sc->view();
+    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    hipStream_t st = (hipStream_t)stream;
+    const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT;
+    if (exh) {
+        if (mt) hipLaunchKernelGGL((k_occluded<true, true>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
+        else hipLaunchKernelGGL((k_occluded<true, false>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
+    } else {
+        if (mt) hipLaunchKernelGGL((k_occluded<false, true>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
+        else hipLaunchKernelGGL((k_occluded<false, false>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
+    }
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int tn_get_counters(tn_scene* sc, uint64_t* out6, int reset) {
+    if (!sc || !out6) return set_error(TN_ERR_INVALID, "NULL argument");
+    unsigned long long c[8];
+    TN_HIP(hipDeviceSynchronize());
+    TN_HIP(hipMemcpy(c, sc->d_counters, sizeof c, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; ++k) out6[k] = c[k];
+    if (reset) TN_HIP(hipMemset(sc->d_counters, 0, sizeof c));
+    return TN_OK;
+}
+
+int tn_tonemap(const float* accum, float* out, uint32_t np, float inv_spp, float exposure,
+               float inverse_gamma, int gamma_enabled, void* stream) {
+    if (!accum || !out) return set_error(TN_ERR_INVALID, "NULL array");
+    if (np == 0) return TN_OK;
+    hipLaunchKernelGGL(k_tonemap, dim3((np + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)accum, (float4*)out, np, inv_spp, exposure, inverse_gamma,
+                       gamma_enabled);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+long tn_write_ppm(const float* rgba, uint32_t w, uint32_t h, char* buf, size_t cap) {
+    if (!rgba || w == 0) return set_error(TN_ERR_INVALID, "bad image");
+    // lib/raster.h:79-100: "P3\n<w> <h>\n255", then per pixel ' ' or '\n'
+    // before "%3d %3d %3d" of int(clamp(255 * c * a)).
+    auto clamp255 = [](float x) { return x < 0 ? 0.f : (255 < x ? 255.f : x); };
+    std::string s;
+    s.reserve((size_t)w * h * 12 + 32);
+    char tmp[64];
+    std::snprintf(tmp, sizeof tmp, "P3\n%u %u\n255", w, h);
+    s += tmp;
+    const size_t n = (size_t)w * h;
+    for (size_t i = 0; i < n; ++i) {
+        s += (i % w == 0) ? '\n' : ' ';
+        const float* c = rgba + 4 * i;
+        std::snprintf(tmp, sizeof tmp, "%3d %3d %3d", (int)clamp255(255 * c[0] * c[3]),
+                      (int)clamp255(255 * c[1] * c[3]), (int)clamp255(255 * c[2] * c[3]));
+        s += tmp;
+    }
+    if (buf) {
+        if (cap < s.size() + 1) return set_error(TN_ERR_LIMIT, "buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    }
+    return (long)s.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,492 @@
+// kdtree_build.cpp -- host SAH kd-tree builder of libturner_amd.
+//
+// Builds the same tree as the reference's KDTree::KDTree (lib/kdtree.cpp:474-490):
+// Wald & Havran 2006 Algorithm 4 with the reference's cost constants, its
+// clipping of triangles at thick box planes, its event order and tie rules,
+// and its flattening.  The arithmetic follows the reference operation by
+// operation (compiled without FMA contraction) so split positions, leaf
+// contents and leaf order are identical; std::sort with the reference's
+// comparator keeps even the order of equal events.  Differences from the
+// reference are in the machinery only: clipping uses fixed-size arrays
+// instead of a heap vector per plane, and independent subtrees are built on
+// parallel host threads (the result does not depend on the thread count).
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <memory>
+#include <thread>
+
+#include "tn_host.h"
+
+namespace tn {
+namespace {
+
+constexpr float kEps = 0.00001f;  // lib/types.h:14
+
+struct V3 {
+    float x, y, z;
+};
+inline V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+inline float at(const V3& a, int ax) { return ax == 0 ? a.x : (ax == 1 ? a.y : a.z); }
+inline void set(V3& a, int ax, float v) { (ax == 0 ? a.x : (ax == 1 ? a.y : a.z)) = v; }
+inline V3 operator-(const V3& a, const V3& b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator+(const V3& a, const V3& b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator*(float s, const V3& v) { return mk(s * v.x, s * v.y, s * v.z); }
+inline float dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 load(const float* p) { return mk(p[0], p[1], p[2]); }
+
+struct Box {
+    V3 lo, hi;
+};
+// src/geometry.h:998-999 (std::min / std::max per component)
+inline Box box_of(const V3& p1, const V3& p2) {
+    return Box{mk(std::min(p1.x, p2.x), std::min(p1.y, p2.y), std::min(p1.z, p2.z)),
+               mk(std::max(p1.x, p2.x), std::max(p1.y, p2.y), std::max(p1.z, p2.z))};
+}
+inline Box box_default() {
+    const float lo = std::numeric_limits<float>::lowest(), hi = std::numeric_limits<float>::max();
+    return Box{mk(lo, lo, lo), mk(hi, hi, hi)};
+}
+inline bool empty(const Box& b) { return b.hi.x <= b.lo.x && b.hi.y <= b.lo.y && b.hi.z <= b.lo.z; }
+inline bool planar(const Box& b, int ax) { return std::abs(at(b.hi, ax) - at(b.lo, ax)) < kEps; }
+inline float area(const Box& b) {
+    V3 d = b.hi - b.lo;
+    return 2 * (d.x * d.y + d.x * d.z + d.y * d.z);
+}
+inline void split(const Box& b, int ax, float pos, Box& l, Box& r) {
+    V3 lmax = b.hi;
+    set(lmax, ax, pos);
+    V3 rmin = b.lo;
+    set(rmin, ax, pos);
+    l = box_of(b.lo, lmax);
+    r = box_of(rmin, b.hi);
+}
+inline float fmin3(float x, float y, float z) { return std::fmin(x, std::min(y, z)); }
+inline float fmax3(float x, float y, float z) { return std::fmax(x, std::max(y, z)); }
+inline Box tri_box(const float* t) {
+    return box_of(mk(fmin3(t[0], t[3], t[6]), fmin3(t[1], t[4], t[7]), fmin3(t[2], t[5], t[8])),
+                  mk(fmax3(t[0], t[3], t[6]), fmax3(t[1], t[4], t[7]), fmax3(t[2], t[5], t[8])));
+}
+
+// Thick-plane Sutherland-Hodgman against the 6 box planes (clipping.h:121-235,
+// intersection.h:17-22), with the reference's full three-term dot products.
+inline int clip_plane(const V3* in, int nin, V3* out, const V3& n, float d) {
+    auto side = [&](const V3& p) {
+        float dist = dot(n, p) - d;
+        return dist > kEps ? 2 : (dist < -kEps ? 1 : 0);
+    };
+    int nout = 0;
+    V3 a = in[nin - 1];
+    int as = side(a);
+    for (int i = 0; i < nin; ++i) {
+        const V3 b = in[i];
+        const int bs = side(b);
+        if (bs == 2) {
+            if (as == 1) {
+                V3 ab = b - a;
+                float t = (d - dot(n, a)) / dot(n, ab);
+                out[nout++] = a + t * (b - a);
+            }
+            out[nout++] = b;
+        } else if (bs == 1) {
+            if (as == 2) {
+                V3 ab = b - a;
+                float t = (d - dot(n, a)) / dot(n, ab);
+                out[nout++] = a + t * (b - a);
+            }
+        } else {
+            out[nout++] = b;
+        }
+        a = b;
+        as = bs;
+    }
+    return nout;
+}
+
+inline Box clip_triangle(const float* t, const Box& box) {
+    V3 buf0[40], buf1[40];
+    V3* pts = buf0;
+    V3* nxt = buf1;
+    int np = 3;
+    pts[0] = load(t);
+    pts[1] = load(t + 3);
+    pts[2] = load(t + 6);
+    for (int ax = 0; ax < 3; ++ax) {
+        for (int s = 0; s < 2; ++s) {
+            V3 n = mk(0, 0, 0);
+            set(n, ax, s == 0 ? 1.f : -1.f);
+            float d = s == 0 ? at(box.lo, ax) : -at(box.hi, ax);
+            np = clip_plane(pts, np, nxt, n, d);
+            std::swap(pts, nxt);
+            if (np < 2) return box_default();
+        }
+    }
+    V3 lo = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+    V3 hi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    for (int ax = 0; ax < 3; ++ax) {
+        for (int i = 0; i < np; ++i) {
+            float c = at(pts[i], ax);
+            if (c < at(lo, ax)) set(lo, ax, c);
+            if (at(hi, ax) < c) set(hi, ax, c);
+        }
+    }
+    return box_of(lo, hi);
+}
+
+enum : int { kEnding = 0, kPlanar = 1, kStarting = 2 };
+struct Event {  // lib/kdtree.cpp:256-265
+    uint32_t id;
+    float point;
+    float point_aux;
+    int type;
+};
+
+struct Node {
+    bool leaf = false;
+    int axis = 0;
+    float split = 0;
+    std::unique_ptr<Node> left, right;
+    std::vector<uint32_t> ids;
+};
+
+inline float lambda_(size_t nl, size_t nr) { return (nl == 0 || nr == 0) ? 0.8f : 1.f; }
+inline float cost_(float lr, float rr, size_t nl, size_t nr) {
+    constexpr int kTrav = 15, kIsect = 20;  // lib/kdtree.cpp:178-179
+    return lambda_(nl, nr) * (kTrav + kIsect * (lr * nl + rr * nr));
+}
+
+class Builder {
+public:
+    Builder(const float* verts, bool parallel) : v_(verts), parallel_(parallel) {
+        unsigned hc = std::thread::hardware_concurrency();
+        budget_.store(parallel ? (int)(hc ? hc : 4) - 1 : 0);
+    }
+
+    std::unique_ptr<Node> build(std::vector<uint32_t> tris, const Box& box) {
+        if (tris.empty()) return nullptr;
+        if (tris.size() <= 3 || area(box) == 0) return leaf(std::move(tris));
+        std::vector<uint32_t> L, R;
+        int ax;
+        float pos;
+        float min_cost = find_plane(tris, box, ax, pos, L, R);
+        if (20 * tris.size() * lambda_(L.size(), R.size()) < min_cost) return leaf(std::move(tris));
+        Box lb, rb;
+        split(box, ax, pos, lb, rb);
+        tris.clear();
+        tris.shrink_to_fit();
+        std::unique_ptr<Node> left, right;
+        const bool fork = parallel_ && L.size() > 4096 && R.size() > 4096 && take_thread();
+        if (fork) {
+            auto fut = std::async(std::launch::async, [this, &L, lb]() {
+                auto r = build(std::move(L), lb);
+                budget_.fetch_add(1);
+                return r;
+            });
+            right = build(std::move(R), rb);
+            left = fut.get();
+        } else {
+            left = build(std::move(L), lb);
+            right = build(std::move(R), rb);
+        }
+        if (!left) return right;
+        if (!right) return left;
+        auto n = std::make_unique<Node>();
+        n->axis = ax;
+        n->split = pos;
+        n->left = std::move(left);
+        n->right = std::move(right);
+        return n;
+    }
+
+private:
+    bool take_thread() {
+        int b = budget_.load();
+        while (b > 0) {
+            if (budget_.compare_exchange_weak(b, b - 1)) return true;
+        }
+        return false;
+    }
+    static std::unique_ptr<Node> leaf(std::vector<uint32_t> ids) {
+        auto n = std::make_unique<Node>();
+        n->leaf = true;
+        n->ids = std::move(ids);
+        return n;
+    }
+
+    // lib/kdtree.cpp:245-408
+    float find_plane(const std::vector<uint32_t>& tris, const Box& box, int& out_ax,
+                     float& out_pos, std::vector<uint32_t>& L, std::vector<uint32_t>& R) const {
+        std::vector<Event> ev[3];
+        for (auto& e : ev) e.reserve(2 * tris.size());
+        size_t num = 0;
+        for (uint32_t id : tris) {
+            Box cb = clip_triangle(v_ + 9 * (size_t)id, box);
+            if (empty(cb)) continue;
+            ++num;
+            for (int a = 0; a < 3; ++a) {
+                if (planar(cb, a)) {
+                    ev[a].push_back(Event{id, at(cb.lo, a), at(cb.lo, a), kPlanar});
+                } else {
+                    ev[a].push_back(Event{id, at(cb.lo, a), at(cb.hi, a), kStarting});
+                    ev[a].push_back(Event{id, at(cb.hi, a), at(cb.lo, a), kEnding});
+                }
+            }
+        }
+        if (num == 0) {
+            out_ax = 0;
+            out_pos = 0;
+            return std::numeric_limits<float>::max();
+        }
+        float min_cost = std::numeric_limits<float>::max();
+        int min_ax = 0;
+        float min_pos = 0;
+        bool min_left = true;
+        const float A = area(box);
+        for (int a = 0; a < 3; ++a) {
+            auto& E = ev[a];
+            std::sort(E.begin(), E.end(), [](const Event& e1, const Event& e2) {
+                return e1.point < e2.point || (e1.point == e2.point && e1.type < e2.type);
+            });
+            size_t nl = 0, np = 0, nr = num;
+            const size_t n = E.size();
+            for (size_t i = 0; i < n;) {
+                const float p = E[i].point;
+                int ps = 0, pe = 0, pp = 0;
+                while (i < n && E[i].point == p && E[i].type == kEnding) { ++pe; ++i; }
+                while (i < n && E[i].point == p && E[i].type == kPlanar) { ++pp; ++i; }
+                while (i < n && E[i].point == p && E[i].type == kStarting) { ++ps; ++i; }
+                np = pp;
+                nr -= pp + pe;
+                Box lb, rb;  // lib/kdtree.cpp:218-238
+                split(box, a, p, lb, rb);
+                float lr = area(lb) / A, rr = area(rb) / A;
+                float lc = cost_(lr, rr, nl + np, nr);
+                float rc = cost_(lr, rr, nl, np + nr);
+                float c = lc < rc ? lc : rc;
+                bool left = lc < rc;
+                if (c < min_cost) {
+                    min_cost = c;
+                    min_ax = a;
+                    min_pos = p;
+                    min_left = left;
+                }
+                nl += ps + pp;
+                np = 0;
+            }
+        }
+        // classify: lib/kdtree.cpp:369-399
+        for (const Event& e : ev[min_ax]) {
+            if (e.point < min_pos) {
+                if (e.type == kEnding || e.type == kPlanar) {
+                    L.push_back(e.id);
+                } else if (min_pos < e.point_aux) {
+                    L.push_back(e.id);
+                    R.push_back(e.id);
+                }
+            } else if (e.point == min_pos) {
+                if (e.type == kEnding) L.push_back(e.id);
+                else if (e.type == kPlanar) (min_left ? L : R).push_back(e.id);
+                else R.push_back(e.id);
+            } else if (e.type == kStarting || e.type == kPlanar) {
+                R.push_back(e.id);
+            }
+        }
+        out_ax = min_ax;
+        out_pos = min_pos;
+        return min_cost;
+    }
+
+    const float* v_;
+    bool parallel_;
+    std::atomic<int> budget_{0};
+};
+
+inline uint64_t flat_inner(int axis, float split, uint32_t right) {
+    uint32_t bits;
+    std::memcpy(&bits, &split, 4);
+    return (uint64_t)bits << 32 | (uint64_t)(right << 2 | (uint32_t)axis);
+}
+inline uint64_t flat_leaf(uint32_t a, uint32_t b) {
+    return (uint64_t)a << 32 | (uint64_t)(uint32_t)(b << 2 | 3u);
+}
+
+// lib/kdtree.cpp:420-467 (depth-first, explicit stack, right pushed first)
+void flatten(const Node* root, std::vector<uint64_t>& out) {
+    constexpr uint32_t kInvalid = 0xFFFFFFFFu >> 2;
+    std::vector<std::pair<const Node*, uint32_t>> st;
+    st.emplace_back(root, kInvalid);
+    while (!st.empty()) {
+        const Node* n = st.back().first;
+        uint32_t parent = st.back().second;
+        st.pop_back();
+        uint32_t me = (uint32_t)out.size();
+        if (parent != kInvalid) {
+            out[parent] = (out[parent] & 0xFFFFFFFF00000000ull) |
+                          (uint64_t)(me << 2 | (uint32_t)(out[parent] & 3));
+        }
+        if (!n->leaf) {
+            out.push_back(flat_inner(n->axis, n->split, kInvalid));
+            st.emplace_back(n->right.get(), me);
+            st.emplace_back(n->left.get(), kInvalid);
+        } else {
+            size_t i = 1;
+            for (; i < n->ids.size(); i += 2) out.push_back(flat_leaf(n->ids[i - 1], n->ids[i]));
+            if (i - 1 < n->ids.size()) out.push_back(flat_leaf(n->ids[i - 1], kInvalid));
+            else out.push_back(flat_inner(0, 0.f, 0));
+        }
+    }
+}
+
+}  // namespace
+
+void precompute_triangles(const float* verts9, uint32_t n, float* out16) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* t = verts9 + 9 * (size_t)i;
+        V3 v0 = load(t), v1 = load(t + 3), v2 = load(t + 6);
+        V3 u = v1 - v0, v = v2 - v0;
+        // src/geometry.h:288-300: the cross product is evaluated in double
+        double ux = u.x, uy = u.y, uz = u.z, vx = v.x, vy = v.y, vz = v.z;
+        V3 c = mk((float)((uy * vz) - (uz * vy)), (float)((uz * vx) - (ux * vz)),
+                  (float)((ux * vy) - (uy * vx)));
+        float len = std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+        V3 nn = (1 / len) * c;  // normalize: (1 / len) * v, src/geometry.h:243-246
+        float uv = dot(u, v), vv = dot(v, v), uu = dot(u, u);
+        float* o = out16 + 16 * (size_t)i;
+        o[0] = v0.x; o[1] = v0.y; o[2] = v0.z;
+        o[3] = nn.x; o[4] = nn.y; o[5] = nn.z;
+        o[6] = u.x; o[7] = u.y; o[8] = u.z;
+        o[9] = v.x; o[10] = v.y; o[11] = v.z;
+        o[12] = uv; o[13] = vv; o[14] = uu; o[15] = uv * uv - uu * vv;
+    }
+}
+
+void build_kdtree(const float* verts9, uint32_t n, bool parallel, std::vector<uint64_t>& flat,
+                  float box6[6]) {
+    std::vector<uint32_t> ids(n, 0);
+    Box box = tri_box(verts9);
+    for (uint32_t i = 1; i < n; ++i) {  // lib/kdtree.cpp:481-485
+        Box b = tri_box(verts9 + 9 * (size_t)i);
+        box = box_of(mk(std::min(box.lo.x, b.lo.x), std::min(box.lo.y, b.lo.y),
+                        std::min(box.lo.z, b.lo.z)),
+                     mk(std::max(box.hi.x, b.hi.x), std::max(box.hi.y, b.hi.y),
+                        std::max(box.hi.z, b.hi.z)));
+        ids[i] = i;
+    }
+    Builder B(verts9, parallel);
+    std::unique_ptr<Node> root = B.build(std::move(ids), box);
+    flat.clear();
+    flatten(root.get(), flat);
+    // tear the tree down iteratively (deep trees would recurse a lot)
+    std::vector<std::unique_ptr<Node>> kill;
+    kill.push_back(std::move(root));
+    while (!kill.empty()) {
+        auto n = std::move(kill.back());
+        kill.pop_back();
+        if (n->left) kill.push_back(std::move(n->left));
+        if (n->right) kill.push_back(std::move(n->right));
+    }
+    box6[0] = box.lo.x; box6[1] = box.lo.y; box6[2] = box.lo.z;
+    box6[3] = box.hi.x; box6[4] = box.hi.y; box6[5] = box.hi.z;
+}
+
+uint32_t tree_height(const std::vector<uint64_t>& flat) {  // lib/kdtree.h:197-216
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 0u}};
+    uint32_t h = 0;
+    while (!st.empty()) {
+        auto [node, level] = st.back();
+        st.pop_back();
+        h = std::max(h, level);
+        uint64_t d = flat[node];
+        if ((d & 3) != 3) {
+            st.emplace_back(node + 1, level + 1);
+            st.emplace_back((uint32_t)d >> 2, level + 1);
+        }
+    }
+    return h;
+}
+
+bool flat_to_device(const std::vector<uint64_t>& flat, uint32_t ntris, std::vector<DevNode>& nodes,
+                    std::vector<uint32_t>& refs, std::string& err) {
+    nodes.clear();
+    refs.clear();
+    if (flat.empty()) {
+        err = "empty tree";
+        return false;
+    }
+    const uint32_t nf = (uint32_t)flat.size();
+    // Pass 1: reachable node starts (inner nodes and leaf-run heads) in DFS order.
+    std::vector<uint32_t> map(nf, 0xFFFFFFFFu);
+    std::vector<uint32_t> order;
+    std::vector<uint32_t> st{0};
+    while (!st.empty()) {
+        uint32_t i = st.back();
+        st.pop_back();
+        if (i >= nf) {
+            err = "node index out of range";
+            return false;
+        }
+        if (map[i] != 0xFFFFFFFFu) {
+            err = "node reached twice";
+            return false;
+        }
+        map[i] = (uint32_t)order.size();
+        order.push_back(i);
+        uint64_t d = flat[i];
+        if ((d & 3) != 3) {
+            uint32_t right = (uint32_t)d >> 2;
+            if (right <= i || right >= nf || i + 1 >= nf) {
+                err = "inner node with bad children";
+                return false;
+            }
+            st.push_back(right);
+            st.push_back(i + 1);
+        }
+    }
+    nodes.resize(order.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+        uint32_t i = order[k];
+        uint64_t d = flat[i];
+        if ((d & 3) != 3) {
+            uint32_t right = map[(uint32_t)d >> 2];
+            nodes[k] = DevNode{(uint32_t)(d >> 32), right << 2 | (uint32_t)(d & 3)};
+        } else {
+            uint32_t off = (uint32_t)refs.size();
+            // lib/kdtree.cpp:599-605: pairs until an inner node or a single-id leaf
+            for (uint32_t j = i;; ++j) {
+                if (j >= nf) {
+                    err = "unterminated leaf run";
+                    return false;
+                }
+                uint64_t e = flat[j];
+                if ((e & 3) != 3) break;
+                uint32_t a = (uint32_t)(e >> 32);
+                if (a >= ntris) {
+                    err = "triangle id out of range";
+                    return false;
+                }
+                refs.push_back(a);
+                if ((uint32_t)e == 0xFFFFFFFFu) break;
+                uint32_t b = (uint32_t)e >> 2;
+                if (b >= ntris) {
+                    err = "triangle id out of range";
+                    return false;
+                }
+                refs.push_back(b);
+            }
+            uint32_t cnt = (uint32_t)refs.size() - off;
+            if (cnt >= (1u << 30)) {
+                err = "leaf too large";
+                return false;
+            }
+            nodes[k] = DevNode{off, cnt << 2 | 3u};
+        }
+    }
+    return true;
+}
+
+}  // namespace tn

@@ -1,0 +1,67 @@
+// tn_scene.h -- the scene handle behind tn_scene* and helpers shared by the
+// HIP translation units of libturner_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tn_host.h"
+#include "tn_traverse.h"
+#include "turner_amd.h"
+
+struct tn_scene {
+    uint32_t ntris = 0;
+    int device = 0;
+    tn::HostTree tree;
+    std::vector<float> pre16;
+    uint32_t nemit = 0;
+    float emit_area = 0.f;
+    // device buffers (owned)
+    uint2* d_nodes = nullptr;
+    uint32_t* d_refs = nullptr;
+    float4* d_tris = nullptr;
+    float* d_normals = nullptr;  // 9 per triangle
+    float4* d_albedo = nullptr;
+    float4* d_emission = nullptr;
+    uint32_t* d_emit_ids = nullptr;
+    float* d_emit_cdf = nullptr;  // inclusive prefix of emitter areas
+    unsigned long long* d_counters = nullptr;  // 8 counters
+    uint32_t* d_status = nullptr;              // [0] incomplete traversals, [1] queue overflow
+
+    tn::SceneView view() const {
+        tn::SceneView v;
+        v.nodes = d_nodes;
+        v.refs = d_refs;
+        v.tris = d_tris;
+        for (int k = 0; k < 3; ++k) {
+            v.bmin[k] = tree.box[k];
+            v.bmax[k] = tree.box[3 + k];
+        }
+        return v;
+    }
+};
+
+namespace tn {
+
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what);
+
+#define TN_HIP(call)                                              \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return ::tn::hip_error(e_, #call);  \
+    } while (0)
+
+// Fixed launch shape of the traversal kernels: 256 lanes = 4 wave64.
+constexpr int kBlock = 256;
+constexpr int kStack = 16;
+
+// Launches closest-hit traversal over n rays (device arrays) into id/r/s/t.
+int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                     float* r, float* s, float* t, uint32_t flags, hipStream_t stream);
+
+}  // namespace tn
