@@ -106,7 +106,8 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
  * triples; outputs id (TN_INVALID_ID on miss), r (ray parameter of the hit,
  * FLT_MAX on miss), s, t (barycentrics of v1, v2; 0 on miss).  Closest hit
  * with the reference's strict-< tie rule in near-first visit order.
- * stream: hipStream_t or NULL for the null stream.  Asynchronous.
+ * stream: hipStream_t or NULL for the null stream.  Asynchronous.  At most
+ * 2^31 rays per call (TN_ERR_LIMIT).
  */
 int tn_intersect(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
                  uint32_t* id, float* r, float* s, float* t, uint32_t flags, void* stream);
@@ -115,7 +116,8 @@ int tn_intersect_host(tn_scene* scene, const float* o3, const float* d3, uint32_
                       uint32_t* id, float* r, float* s, float* t, uint32_t flags);
 /* Shadow / visibility query with the reference's predicate
  * (pathtracer.cpp:49-53): occluded[i] = 1 iff some triangle is hit at
- * 0 <= r < tmax[i].  Device pointers.  Asynchronous. */
+ * 0 <= r <= tmax[i], i.e. iff !(!hit || r_closest > tmax[i]).  Device
+ * pointers.  Asynchronous. */
 int tn_occluded(tn_scene* scene, const float* o3, const float* d3, const float* tmax,
                 uint32_t n, uint8_t* occluded, uint32_t flags, void* stream);
 /* Work counters of the launches made with TN_ISECT_COUNT since the last reset:

@@ -80,7 +80,7 @@ def test_icosphere_bit_exact(gpu, level):
     sc = scenes.intersect_scene(level=level)
     tree = T.KDTree(sc["verts"])
     rng = np.random.default_rng(level)
-    o, d = scenes.random_rays(1 << 17, seed=level)
+    o, d = scenes.random_rays((1 << 17) + 12345, seed=level)  # not a multiple of the chunk
     ho, hd = H.hard_rays(sc["verts"], 1 << 15, rng)
     o = np.concatenate([o, ho])
     d = np.concatenate([d, hd])

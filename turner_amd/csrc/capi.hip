@@ -34,63 +34,87 @@ int hip_error(hipError_t e, const char* what) {
 
 namespace {
 
-// Batched closest hit: one ray per lane, LDS short stack of kStack entries.
-template <bool EXH, bool MT, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_intersect(SceneView sc, const float* __restrict__ o3,
-                                                      const float* __restrict__ d3, uint32_t n,
-                                                      uint32_t* __restrict__ out_id,
-                                                      float* __restrict__ out_r,
-                                                      float* __restrict__ out_s,
-                                                      float* __restrict__ out_t,
-                                                      unsigned long long* counters,
-                                                      uint32_t* status) {
+// Batched traversal with lane refill: each wave owns a chunk of kChunk rays
+// and, after every leaf step, hands lanes whose ray finished (or missed the
+// root box) the chunk's next rays (ballot + mbcnt), so a wave is not held by
+// its slowest ray.  ANY = visibility query against tmax (tn_occluded).
+template <bool EXH, bool MT, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __restrict__ o3,
+                                                 const float* __restrict__ d3,
+                                                 const float* __restrict__ tmax, uint32_t n,
+                                                 uint32_t* __restrict__ out_id,
+                                                 float* __restrict__ out_r,
+                                                 float* __restrict__ out_s,
+                                                 float* __restrict__ out_t,
+                                                 uint8_t* __restrict__ occ,
+                                                 unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
     __shared__ uint32_t red[6];
     if (COUNT && threadIdx.x < 6) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint2* my = stk + threadIdx.x;
+    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    uint32_t next = wave * (uint32_t)kChunk;
+    const uint32_t end = next < n ? min(next + (uint32_t)kChunk, n) : next;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t ray = kNone;
+    Trav T;
+    float tmx = 0.f;
     TraversalCounters cnt;
-    if (i < n) {
-        const float3 o = make_float3(o3[3 * (size_t)i], o3[3 * (size_t)i + 1], o3[3 * (size_t)i + 2]);
-        const float3 d = make_float3(d3[3 * (size_t)i], d3[3 * (size_t)i + 1], d3[3 * (size_t)i + 2]);
-        float r, s, t;
-        const uint32_t id = traverse<kStack, EXH, MT, false, COUNT>(sc, o, d, 0.f, r, s, t,
-                                                                    stk + threadIdx.x, kBlock, cnt);
-        const bool hit = id != kInvalidId;
-        out_id[i] = id;
-        out_r[i] = hit ? r : FLT_MAX;
-        out_s[i] = hit ? s : 0.f;
-        out_t[i] = hit ? t : 0.f;
-        if (cnt.incomplete) atomicAdd(status, 1u);
+    uint32_t nrays = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(ray == kNone);
+        if (idle && next < end) {
+            const uint32_t k = next + lanes_below(idle);
+            next += (uint32_t)__popcll(idle);
+            if (ray == kNone && k < end) {
+                const float3 o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
+                const float3 d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
+                if (ANY) tmx = tmax[k];
+                if (COUNT) nrays++;
+                if (trav_init(T, sc, o, d)) {
+                    ray = k;
+                } else if (ANY) {
+                    occ[k] = 0;
+                } else {
+                    out_id[k] = kInvalidId;
+                    out_r[k] = FLT_MAX;
+                    out_s[k] = 0.f;
+                    out_t[k] = 0.f;
+                }
+            }
+        }
+        if (__ballot(ray != kNone) == 0) {
+            if (next >= end) break;
+            continue;
+        }
+        if (ray != kNone) {
+            if (trav_step<kStack, EXH, MT, ANY, COUNT>(T, sc, tmx, my, kBlock, cnt)) {
+                const bool hit = T.best_id != kInvalidId;
+                if (ANY) {
+                    occ[ray] = hit ? 1 : 0;
+                } else {
+                    out_id[ray] = T.best_id;
+                    out_r[ray] = hit ? T.best_r : FLT_MAX;
+                    out_s[ray] = hit ? T.best_s : 0.f;
+                    out_t[ray] = hit ? T.best_t : 0.f;
+                }
+                if (COUNT) cnt.restarts += T.restarts;
+                ray = kNone;
+            }
+        }
     }
+    if (cnt.incomplete) atomicAdd(status, 1u);
     if (COUNT) {
         atomicAdd(&red[0], cnt.inner);
         atomicAdd(&red[1], cnt.leaves);
         atomicAdd(&red[2], cnt.refs);
         atomicAdd(&red[3], cnt.tests);
-        atomicAdd(&red[4], i < n ? 1u : 0u);
+        atomicAdd(&red[4], nrays);
         atomicAdd(&red[5], cnt.restarts);
         __syncthreads();
         if (threadIdx.x < 6) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
     }
-}
-
-template <bool EXH, bool MT>
-__global__ __launch_bounds__(kBlock) void k_occluded(SceneView sc, const float* __restrict__ o3,
-                                                     const float* __restrict__ d3,
-                                                     const float* __restrict__ tmax, uint32_t n,
-                                                     uint8_t* __restrict__ occ, uint32_t* status) {
-    __shared__ uint2 stk[kStack * kBlock];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float3 o = make_float3(o3[3 * (size_t)i], o3[3 * (size_t)i + 1], o3[3 * (size_t)i + 2]);
-    const float3 d = make_float3(d3[3 * (size_t)i], d3[3 * (size_t)i + 1], d3[3 * (size_t)i + 2]);
-    TraversalCounters cnt;
-    float r, s, t;
-    const uint32_t id = traverse<kStack, EXH, MT, true, false>(sc, o, d, tmax[i], r, s, t,
-                                                               stk + threadIdx.x, kBlock, cnt);
-    occ[i] = id != kInvalidId ? 1 : 0;
-    if (cnt.incomplete) atomicAdd(status, 1u);
 }
 
 __global__ void k_tonemap(const float4* __restrict__ acc, float4* __restrict__ out, uint32_t n,
@@ -225,35 +249,43 @@ int prepare_triangles(tn_scene* sc, const float* verts9, uint32_t n) {
 
 }  // namespace
 
-int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
-                     float* r, float* s, float* t, uint32_t flags, hipStream_t stream) {
+int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
+                uint32_t* id, float* r, float* s, float* t, uint8_t* occ, uint32_t flags,
+                hipStream_t stream) {
     if (n == 0) return TN_OK;
+    if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
     const SceneView v = sc->view();
-    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    const uint32_t waves = (n + kChunk - 1) / kChunk;
+    const dim3 grid((waves + kBlock / 64 - 1) / (kBlock / 64)), block(kBlock);
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
-               cnt = flags & TN_ISECT_COUNT;
+               cnt = flags & TN_ISECT_COUNT, any = occ != nullptr;
     auto go = [&](auto kern) {
-        kern<<<grid, block, 0, stream>>>(v, o3, d3, n, id, r, s, t, sc->d_counters, sc->d_status);
+        kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, id, r, s, t, occ, sc->d_counters,
+                                         sc->d_status);
     };
-    if (cnt) {
-        if (exh) {
-            if (mt) go(k_intersect<true, true, true>);
-            else go(k_intersect<true, false, true>);
-        } else {
-            if (mt) go(k_intersect<false, true, true>);
-            else go(k_intersect<false, false, true>);
-        }
-    } else {
-        if (exh) {
-            if (mt) go(k_intersect<true, true, false>);
-            else go(k_intersect<true, false, false>);
-        } else {
-            if (mt) go(k_intersect<false, true, false>);
-            else go(k_intersect<false, false, false>);
-        }
+#define TN_SEL(A, C)                                                     \
+    if (exh) {                                                           \
+        if (mt) go(k_rays<true, true, A, C>);                            \
+        else go(k_rays<true, false, A, C>);                              \
+    } else {                                                             \
+        if (mt) go(k_rays<false, true, A, C>);                           \
+        else go(k_rays<false, false, A, C>);                             \
     }
+    if (any) {
+        TN_SEL(true, false)
+    } else if (cnt) {
+        TN_SEL(false, true)
+    } else {
+        TN_SEL(false, false)
+    }
+#undef TN_SEL
     TN_HIP(hipGetLastError());
     return TN_OK;
+}
+
+int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                     float* r, float* s, float* t, uint32_t flags, hipStream_t stream) {
+    return launch_rays(sc, o3, d3, nullptr, n, id, r, s, t, nullptr, flags, stream);
 }
 
 }  // namespace tn
@@ -419,19 +451,8 @@ int tn_occluded(tn_scene* sc, const float* o3, const float* d3, const float* tma
     if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
     if (n == 0) return TN_OK;
     if (!o3 || !d3 || !tmax || !occ) return set_error(TN_ERR_INVALID, "NULL array");
-    const SceneView v = sc->view();
-    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
-    hipStream_t st = (hipStream_t)stream;
-    const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT;
-    if (exh) {
-        if (mt) hipLaunchKernelGGL((k_occluded<true, true>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
-        else hipLaunchKernelGGL((k_occluded<true, false>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
-    } else {
-        if (mt) hipLaunchKernelGGL((k_occluded<false, true>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
-        else hipLaunchKernelGGL((k_occluded<false, false>), grid, block, 0, st, v, o3, d3, tmax, n, occ, sc->d_status);
-    }
-    TN_HIP(hipGetLastError());
-    return TN_OK;
+    return launch_rays(sc, o3, d3, tmax, n, nullptr, nullptr, nullptr, nullptr, occ,
+                       flags & ~TN_ISECT_COUNT, (hipStream_t)stream);
 }
 
 int tn_get_counters(tn_scene* sc, uint64_t* out6, int reset) {

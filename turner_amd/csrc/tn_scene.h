@@ -58,8 +58,14 @@ int hip_error(hipError_t e, const char* what);
 
 // Fixed launch shape of the traversal kernels: 256 lanes = 4 wave64.
 constexpr int kBlock = 256;
-constexpr int kStack = 16;
+constexpr int kStack = 8;
+constexpr int kChunk = 512;  // rays per wave in the refill kernels
 
+// Traversal over n rays (device arrays): closest hit into id/r/s/t, or with
+// occ != NULL visibility against tmax into occ.
+int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
+                uint32_t* id, float* r, float* s, float* t, uint8_t* occ, uint32_t flags,
+                hipStream_t stream);
 // Launches closest-hit traversal over n rays (device arrays) into id/r/s/t.
 int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
                      float* r, float* s, float* t, uint32_t flags, hipStream_t stream);

@@ -22,9 +22,11 @@
 //     leaf processed, so it is not stored.  When entries are lost to
 //     overflow, the traversal restarts from the root at the last leaf's exit
 //     (kd-restart), which visits the remaining leaves in the same order;
-//   * near-first early exit: the loop ends once the best hit lies before the
+//   * near-first early exit: traversal ends once the best hit lies before the
 //     current leaf's exit by a 2^-16 relative margin (no later leaf can then
-//     hold a strictly closer hit).  TN_ISECT_EXHAUSTIVE disables it.
+//     hold a strictly closer hit).  TN_ISECT_EXHAUSTIVE disables it;
+//   * the traversal is a per-lane state machine (trav_init / trav_step, one
+//     leaf per step) so a wave can hand finished lanes new rays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,7 +37,7 @@
 namespace tn {
 
 constexpr uint32_t kInvalidId = 1u << 30;
-constexpr float kDirEps = 0.00001f;             // lib/types.h:14
+constexpr float kDirEps = 0.00001f;  // lib/types.h:14
 constexpr float kExitMargin = 1.0f + 1.0f / 65536.0f;
 constexpr int kMaxRestarts = 4096;
 
@@ -52,29 +54,56 @@ struct TraversalCounters {
     bool incomplete = false;
 };
 
+struct Trav {
+    float3 o, d, fd, dinv;
+    float tenter, texit, root_exit;
+    uint32_t node;
+    int head, count, restarts;
+    bool lost;
+    uint32_t best_id;
+    float best_r, best_s, best_t;
+};
+
 __device__ __forceinline__ float pick(float3 v, uint32_t ax) {
     return ax == 0 ? v.x : (ax == 1 ? v.y : v.z);
 }
 
+// True when num / den is certainly < 0: opposite signs and a quotient that
+// cannot underflow to -0 (|num / den| >= 2^-100).  Lets a test reject
+// without the division while giving exactly the division's verdict.
+__device__ __forceinline__ bool certainly_negative(float num, float den) {
+    return ((num < 0.f) != (den < 0.f)) && num != 0.f &&
+           fabsf(num) >= fabsf(den) * 0x1p-100f;
+}
+
 // lib/intersection.h:40-49 + 63-89: plane, then barycentrics; no culling.
+// Returns false also when r is not below `bound` (closest hit: the current
+// best, which a candidate must beat strictly) or, with ANY, above it.
+template <bool ANY>
 __device__ __forceinline__ bool leaf_test_ref(const float4* __restrict__ T, uint32_t id, float3 o,
-                                              float3 d, float& r, float& s, float& t) {
+                                              float3 d, float bound, float& r, float& s, float& t) {
     const float4 a = T[4 * (size_t)id];      // v0, uv
     const float4 b = T[4 * (size_t)id + 1];  // n, vv
     const float den = b.x * d.x + b.y * d.y + b.z * d.z;
     if (den == 0.f) return false;  // r = lowest() < 0
     const float nom = b.x * (a.x - o.x) + b.y * (a.y - o.y) + b.z * (a.z - o.z);
+    if (certainly_negative(nom, den)) return false;
     const float rr = nom / den;
     if (rr < 0) return false;
+    if (ANY ? !(rr <= bound) : !(rr < bound)) return false;
     const float4 c = T[4 * (size_t)id + 2];  // u, uu
     const float4 e = T[4 * (size_t)id + 3];  // v, denom
     const float px = o.x + rr * d.x, py = o.y + rr * d.y, pz = o.z + rr * d.z;
     const float wx = px - a.x, wy = py - a.y, wz = pz - a.z;
     const float wv = wx * e.x + wy * e.y + wz * e.z;
     const float wu = wx * c.x + wy * c.y + wz * c.z;
-    const float ss = (a.w * wv - b.w * wu) / e.w;
+    const float snum = a.w * wv - b.w * wu;
+    if (certainly_negative(snum, e.w)) return false;
+    const float ss = snum / e.w;
     if (ss < 0) return false;
-    const float tt = (a.w * wu - c.w * wv) / e.w;
+    const float tnum = a.w * wu - c.w * wv;
+    if (certainly_negative(tnum, e.w)) return false;
+    const float tt = tnum / e.w;
     if (tt < 0 || 1 < ss + tt) return false;
     r = rr;
     s = ss;
@@ -83,8 +112,9 @@ __device__ __forceinline__ bool leaf_test_ref(const float4* __restrict__ T, uint
 }
 
 // Moeller & Trumbore 1997 in FP32 on the record's v0, e1 = u, e2 = v.
+template <bool ANY>
 __device__ __forceinline__ bool leaf_test_mt(const float4* __restrict__ T, uint32_t id, float3 o,
-                                             float3 d, float& r, float& s, float& t) {
+                                             float3 d, float bound, float& r, float& s, float& t) {
     const float4 a = T[4 * (size_t)id];
     const float4 c = T[4 * (size_t)id + 2];
     const float4 e = T[4 * (size_t)id + 3];
@@ -100,124 +130,146 @@ __device__ __forceinline__ bool leaf_test_mt(const float4* __restrict__ T, uint3
     if (v < 0.f || u + v > 1.f) return false;
     const float tt = (e.x * qx + e.y * qy + e.z * qz) * inv;
     if (tt < 0.f) return false;
+    if (ANY ? !(tt <= bound) : !(tt < bound)) return false;
     r = tt;
     s = u;
     t = v;
     return true;
 }
 
-// Closest hit (ANY = false) or visibility up to tmax_any (ANY = true, returns
-// the id of some triangle hit at 0 <= r <= tmax_any).  stk points at this
-// lane's first LDS stack slot; consecutive slots are `stride` entries apart.
-template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
-__device__ __forceinline__ uint32_t traverse(const SceneView& sc, float3 o, float3 d, float tmax_any,
-                                             float& r_out, float& s_out, float& t_out, uint2* stk,
-                                             int stride, TraversalCounters& cnt) {
-    static_assert((S & (S - 1)) == 0, "stack size must be a power of two");
+// Fixes the direction and clips the ray to the root box (kdtree.cpp:519-535).
+// Returns false when the root box is missed.
+__device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o, float3 d) {
+    T.o = o;
+    T.d = d;
     float3 fd = d;
     if (fd.x == 0.f) fd.x = kDirEps;
     if (fd.y == 0.f) fd.y = kDirEps;
     if (fd.z == 0.f) fd.z = kDirEps;
-    const float3 dinv = make_float3(1 / fd.x, 1 / fd.y, 1 / fd.z);
-    float tenter, texit;
-    {
-        const float tx1 = (sc.bmin[0] - o.x) * dinv.x, tx2 = (sc.bmax[0] - o.x) * dinv.x;
-        float lo = fminf(tx1, tx2), hi = fmaxf(tx1, tx2);
-        const float ty1 = (sc.bmin[1] - o.y) * dinv.y, ty2 = (sc.bmax[1] - o.y) * dinv.y;
-        lo = fmaxf(lo, fminf(ty1, ty2));
-        hi = fminf(hi, fmaxf(ty1, ty2));
-        const float tz1 = (sc.bmin[2] - o.z) * dinv.z, tz2 = (sc.bmax[2] - o.z) * dinv.z;
-        lo = fmaxf(lo, fminf(tz1, tz2));
-        hi = fminf(hi, fmaxf(tz1, tz2));
-        if (hi < lo) return kInvalidId;
-        tenter = lo;
-        texit = hi;
+    T.fd = fd;
+    T.dinv = make_float3(1 / fd.x, 1 / fd.y, 1 / fd.z);
+    const float3 di = T.dinv;
+    const float tx1 = (sc.bmin[0] - o.x) * di.x, tx2 = (sc.bmax[0] - o.x) * di.x;
+    float lo = fminf(tx1, tx2), hi = fmaxf(tx1, tx2);
+    const float ty1 = (sc.bmin[1] - o.y) * di.y, ty2 = (sc.bmax[1] - o.y) * di.y;
+    lo = fmaxf(lo, fminf(ty1, ty2));
+    hi = fminf(hi, fmaxf(ty1, ty2));
+    const float tz1 = (sc.bmin[2] - o.z) * di.z, tz2 = (sc.bmax[2] - o.z) * di.z;
+    lo = fmaxf(lo, fminf(tz1, tz2));
+    hi = fminf(hi, fmaxf(tz1, tz2));
+    T.tenter = lo;
+    T.texit = hi;
+    T.root_exit = hi;
+    T.node = 0;
+    T.head = 0;
+    T.count = 0;
+    T.restarts = 0;
+    T.lost = false;
+    T.best_id = kInvalidId;
+    T.best_r = FLT_MAX;
+    T.best_s = 0.f;
+    T.best_t = 0.f;
+    return !(hi < lo);
+}
+
+// Descends from T.node to the next leaf, tests it and moves to the next
+// stack entry.  Returns true when the traversal is complete.
+template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tmax_any, uint2* stk,
+                                          int stride, TraversalCounters& cnt) {
+    static_assert((S & (S - 1)) == 0, "stack size must be a power of two");
+    uint32_t node = T.node;
+    float tenter = T.tenter, texit = T.texit;
+    uint2 nd = sc.nodes[node];
+    while ((nd.y & 3u) != 3u) {
+        if (COUNT) cnt.inner++;
+        const uint32_t ax = nd.y & 3u;
+        const float tt = (__uint_as_float(nd.x) - pick(T.o, ax)) * pick(T.dinv, ax);
+        uint32_t nearn = node + 1, farn = nd.y >> 2;
+        if (pick(T.fd, ax) <= 0.f) {
+            const uint32_t tmp = nearn;
+            nearn = farn;
+            farn = tmp;
+        }
+        if (texit < tt) {
+            node = nearn;
+        } else if (tt < tenter) {
+            node = farn;
+        } else {
+            stk[T.head * stride] = make_uint2(farn, __float_as_uint(texit));
+            T.head = (T.head + 1) & (S - 1);
+            if (T.count < S) ++T.count;
+            else T.lost = true;
+            node = nearn;
+            texit = tt;
+        }
+        nd = sc.nodes[node];
     }
-    const float root_exit = texit;
-    uint32_t node = 0;
-    int head = 0, count = 0, restarts = 0;
-    bool lost = false;
-    uint32_t best_id = kInvalidId;
-    float best_r = FLT_MAX, best_s = 0.f, best_t = 0.f;
-    for (;;) {
-        uint2 nd = sc.nodes[node];
-        while ((nd.y & 3u) != 3u) {
-            if (COUNT) cnt.inner++;
-            const uint32_t ax = nd.y & 3u;
-            const float tt = (__uint_as_float(nd.x) - pick(o, ax)) * pick(dinv, ax);
-            uint32_t nearn = node + 1, farn = nd.y >> 2;
-            if (pick(fd, ax) <= 0.f) {
-                const uint32_t tmp = nearn;
-                nearn = farn;
-                farn = tmp;
-            }
-            if (texit < tt) {
-                node = nearn;
-            } else if (tt < tenter) {
-                node = farn;
-            } else {
-                stk[head * stride] = make_uint2(farn, __float_as_uint(texit));
-                head = (head + 1) & (S - 1);
-                if (count < S) ++count;
-                else lost = true;
-                node = nearn;
-                texit = tt;
-            }
-            nd = sc.nodes[node];
+    if (COUNT) cnt.leaves++;
+    const uint32_t off = nd.x, nref = nd.y >> 2;
+    for (uint32_t k = 0; k < nref; ++k) {
+        const uint32_t id = sc.refs[off + k];
+        float rr, ss, tt;
+        const float bound = ANY ? tmax_any : T.best_r;
+        const bool hit = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
+                            : leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt);
+        if (COUNT) {
+            cnt.refs++;
+            cnt.tests++;
         }
-        if (COUNT) cnt.leaves++;
-        const uint32_t off = nd.x, nref = nd.y >> 2;
-        for (uint32_t k = 0; k < nref; ++k) {
-            const uint32_t id = sc.refs[off + k];
-            float rr, ss, tt;
-            const bool hit = MT ? leaf_test_mt(sc.tris, id, o, d, rr, ss, tt)
-                                : leaf_test_ref(sc.tris, id, o, d, rr, ss, tt);
-            if (COUNT) {
-                cnt.refs++;
-                cnt.tests++;
-            }
-            if (ANY) {
-                if (hit && rr <= tmax_any) {
-                    r_out = rr;
-                    s_out = ss;
-                    t_out = tt;
-                    return id;
-                }
-            } else if (hit && rr < best_r) {
-                best_r = rr;
-                best_s = ss;
-                best_t = tt;
-                best_id = id;
-            }
+        if (hit) {
+            T.best_r = rr;
+            T.best_s = ss;
+            T.best_t = tt;
+            T.best_id = id;
+            if (ANY) return true;
         }
-        if (!EXH) {
-            if (!ANY && best_id != kInvalidId && best_r * kExitMargin < texit) break;
-            if (ANY && tmax_any * kExitMargin < texit) break;
-        }
-        if (count == 0) {
-            if (!lost || !(texit < root_exit)) break;
-            if (++restarts > kMaxRestarts) {
-                cnt.incomplete = true;
-                break;
-            }
-            node = 0;
-            tenter = texit;
-            texit = root_exit;
-            lost = false;
-            continue;
-        }
-        head = (head - 1) & (S - 1);
-        --count;
-        const uint2 e = stk[head * stride];
-        node = e.x;
-        tenter = texit;
-        texit = __uint_as_float(e.y);
     }
-    if (COUNT) cnt.restarts += restarts;
-    r_out = best_r;
-    s_out = best_s;
-    t_out = best_t;
-    return best_id;
+    if (!EXH) {
+        if (!ANY && T.best_id != kInvalidId && T.best_r * kExitMargin < texit) return true;
+        if (ANY && tmax_any * kExitMargin < texit) return true;
+    }
+    if (T.count == 0) {
+        if (!T.lost || !(texit < T.root_exit)) return true;
+        if (++T.restarts > kMaxRestarts) {
+            cnt.incomplete = true;
+            return true;
+        }
+        T.node = 0;
+        T.tenter = texit;
+        T.texit = T.root_exit;
+        T.lost = false;
+        return false;
+    }
+    T.head = (T.head - 1) & (S - 1);
+    --T.count;
+    const uint2 e = stk[T.head * stride];
+    T.node = e.x;
+    T.tenter = texit;
+    T.texit = __uint_as_float(e.y);
+    return false;
+}
+
+// Whole traversal of one ray (closest hit, or visibility up to tmax_any).
+template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
+__device__ __forceinline__ uint32_t traverse(const SceneView& sc, float3 o, float3 d, float tmax_any,
+                                             float& r_out, float& s_out, float& t_out, uint2* stk,
+                                             int stride, TraversalCounters& cnt) {
+    Trav T;
+    if (!trav_init(T, sc, o, d)) return kInvalidId;
+    while (!trav_step<S, EXH, MT, ANY, COUNT>(T, sc, tmax_any, stk, stride, cnt)) {
+    }
+    if (COUNT) cnt.restarts += T.restarts;
+    r_out = T.best_r;
+    s_out = T.best_s;
+    t_out = T.best_t;
+    return T.best_id;
+}
+
+// Wave64 helpers for lane refill.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 }  // namespace tn
