@@ -122,8 +122,10 @@ int tn_occluded(tn_scene* scene, const float* o3, const float* d3, const float* 
                 uint32_t n, uint8_t* occluded, uint32_t flags, void* stream);
 /* Work counters of the launches made with TN_ISECT_COUNT since the last reset:
  * [0] inner nodes visited, [1] leaves visited, [2] triangle references read,
- * [3] ray-triangle tests, [4] rays, [5] stack restarts. reset != 0 zeroes them. */
-int tn_get_counters(tn_scene* scene, uint64_t* out6, int reset);
+ * [3] ray-triangle tests, [4] rays, [5] stack restarts, [6] tests that read
+ * the whole 64-byte triangle record (the rest stop after 32 bytes), [7] 0.
+ * reset != 0 zeroes them.  Synchronises the device. */
+int tn_get_counters(tn_scene* scene, uint64_t* out8, int reset);
 
 /* Camera as lib/types.h:89-144: ray direction =
  * trafo * (-delta_x (1 - 2 px / w), delta_y (1 - 2 py / h), -1), origin pos. */

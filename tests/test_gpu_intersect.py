@@ -193,14 +193,14 @@ def test_torch_device_batch_and_counters(gpu):
     o, d = scenes.random_rays(1 << 16, seed=8)
     dev = torch.device("cuda:0")
     isect = T.KDTreeIntersection(tree)
-    T.check(T.lib().tn_get_counters(tree.handle, np.zeros(6, np.uint64).ctypes.data, 1))
+    T.check(T.lib().tn_get_counters(tree.handle, np.zeros(8, np.uint64).ctypes.data, 1))
     ids, r, s, t = isect.intersect_batch(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev),
                                          TN_ISECT_COUNT)
     torch.cuda.synchronize()
     ref = oracle_hits(tree, o, d)
     assert_bit_exact((ids.cpu().numpy().view(np.uint32), r.cpu().numpy(), s.cpu().numpy(),
                       t.cpu().numpy()), ref, "torch")
-    cnt = np.zeros(6, np.uint64)
+    cnt = np.zeros(8, np.uint64)
     T.check(T.lib().tn_get_counters(tree.handle, cnt.ctypes.data, 1))
     assert cnt[4] == len(o)
     oc = O.intersect_count(tree.triangle_records(), tree.flat_nodes(), tree.box(), o, d)

@@ -49,11 +49,16 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
                                                  uint8_t* __restrict__ occ,
                                                  unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
-    __shared__ uint32_t red[6];
-    if (COUNT && threadIdx.x < 6) red[threadIdx.x] = 0;
+    __shared__ uint32_t red[7];
+    if (COUNT && threadIdx.x < 7) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
     uint2* my = stk + threadIdx.x;
-    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of chunks so rays that are adjacent in the batch share
+    // an L2.  The grid is a multiple of 8 blocks, so this is a bijection.
+    const uint32_t per_xcd = gridDim.x / kXcds;
+    const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
     uint32_t next = wave * (uint32_t)kChunk;
     const uint32_t end = next < n ? min(next + (uint32_t)kChunk, n) : next;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -112,8 +117,9 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
         atomicAdd(&red[3], cnt.tests);
         atomicAdd(&red[4], nrays);
         atomicAdd(&red[5], cnt.restarts);
+        atomicAdd(&red[6], cnt.full);
         __syncthreads();
-        if (threadIdx.x < 6) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+        if (threadIdx.x < 7) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
     }
 }
 
@@ -256,7 +262,8 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
     const SceneView v = sc->view();
     const uint32_t waves = (n + kChunk - 1) / kChunk;
-    const dim3 grid((waves + kBlock / 64 - 1) / (kBlock / 64)), block(kBlock);
+    const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
                cnt = flags & TN_ISECT_COUNT, any = occ != nullptr;
     auto go = [&](auto kern) {
@@ -455,12 +462,12 @@ int tn_occluded(tn_scene* sc, const float* o3, const float* d3, const float* tma
                        flags & ~TN_ISECT_COUNT, (hipStream_t)stream);
 }
 
-int tn_get_counters(tn_scene* sc, uint64_t* out6, int reset) {
-    if (!sc || !out6) return set_error(TN_ERR_INVALID, "NULL argument");
+int tn_get_counters(tn_scene* sc, uint64_t* out8, int reset) {
+    if (!sc || !out8) return set_error(TN_ERR_INVALID, "NULL argument");
     unsigned long long c[8];
     TN_HIP(hipDeviceSynchronize());
     TN_HIP(hipMemcpy(c, sc->d_counters, sizeof c, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 6; ++k) out6[k] = c[k];
+    for (int k = 0; k < 8; ++k) out8[k] = c[k];
     if (reset) TN_HIP(hipMemset(sc->d_counters, 0, sizeof c));
     return TN_OK;
 }

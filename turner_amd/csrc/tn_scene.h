@@ -60,6 +60,7 @@ int hip_error(hipError_t e, const char* what);
 constexpr int kBlock = 256;
 constexpr int kStack = 8;
 constexpr int kChunk = 512;  // rays per wave in the refill kernels
+constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, one L2 each
 
 // Traversal over n rays (device arrays): closest hit into id/r/s/t, or with
 // occ != NULL visibility against tmax into occ.

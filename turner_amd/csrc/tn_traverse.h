@@ -51,6 +51,7 @@ struct SceneView {
 
 struct TraversalCounters {
     uint32_t inner = 0, leaves = 0, refs = 0, tests = 0, restarts = 0;
+    uint32_t full = 0;  // tests that read the whole 64-byte record
     bool incomplete = false;
 };
 
@@ -77,20 +78,22 @@ __device__ __forceinline__ bool certainly_negative(float num, float den) {
 }
 
 // lib/intersection.h:40-49 + 63-89: plane, then barycentrics; no culling.
-// Returns false also when r is not below `bound` (closest hit: the current
-// best, which a candidate must beat strictly) or, with ANY, above it.
+// A candidate is also rejected when r is not below `bound` (closest hit: the
+// current best, which it must beat strictly) or, with ANY, above it.
+// Returns 2 on a hit, 1 for a miss decided after reading the whole record,
+// 0 for a miss decided from its first 32 bytes.
 template <bool ANY>
-__device__ __forceinline__ bool leaf_test_ref(const float4* __restrict__ T, uint32_t id, float3 o,
-                                              float3 d, float bound, float& r, float& s, float& t) {
+__device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint32_t id, float3 o,
+                                             float3 d, float bound, float& r, float& s, float& t) {
     const float4 a = T[4 * (size_t)id];      // v0, uv
     const float4 b = T[4 * (size_t)id + 1];  // n, vv
     const float den = b.x * d.x + b.y * d.y + b.z * d.z;
-    if (den == 0.f) return false;  // r = lowest() < 0
+    if (den == 0.f) return 0;  // r = lowest() < 0
     const float nom = b.x * (a.x - o.x) + b.y * (a.y - o.y) + b.z * (a.z - o.z);
-    if (certainly_negative(nom, den)) return false;
+    if (certainly_negative(nom, den)) return 0;
     const float rr = nom / den;
-    if (rr < 0) return false;
-    if (ANY ? !(rr <= bound) : !(rr < bound)) return false;
+    if (rr < 0) return 0;
+    if (ANY ? !(rr <= bound) : !(rr < bound)) return 0;
     const float4 c = T[4 * (size_t)id + 2];  // u, uu
     const float4 e = T[4 * (size_t)id + 3];  // v, denom
     const float px = o.x + rr * d.x, py = o.y + rr * d.y, pz = o.z + rr * d.z;
@@ -98,43 +101,44 @@ __device__ __forceinline__ bool leaf_test_ref(const float4* __restrict__ T, uint
     const float wv = wx * e.x + wy * e.y + wz * e.z;
     const float wu = wx * c.x + wy * c.y + wz * c.z;
     const float snum = a.w * wv - b.w * wu;
-    if (certainly_negative(snum, e.w)) return false;
+    if (certainly_negative(snum, e.w)) return 1;
     const float ss = snum / e.w;
-    if (ss < 0) return false;
+    if (ss < 0) return 1;
     const float tnum = a.w * wu - c.w * wv;
-    if (certainly_negative(tnum, e.w)) return false;
+    if (certainly_negative(tnum, e.w)) return 1;
     const float tt = tnum / e.w;
-    if (tt < 0 || 1 < ss + tt) return false;
+    if (tt < 0 || 1 < ss + tt) return 1;
     r = rr;
     s = ss;
     t = tt;
-    return true;
+    return 2;
 }
 
 // Moeller & Trumbore 1997 in FP32 on the record's v0, e1 = u, e2 = v.
+// Returns 2 on a hit, 1 on a miss (the test always reads v0, u and v).
 template <bool ANY>
-__device__ __forceinline__ bool leaf_test_mt(const float4* __restrict__ T, uint32_t id, float3 o,
-                                             float3 d, float bound, float& r, float& s, float& t) {
+__device__ __forceinline__ int leaf_test_mt(const float4* __restrict__ T, uint32_t id, float3 o,
+                                            float3 d, float bound, float& r, float& s, float& t) {
     const float4 a = T[4 * (size_t)id];
     const float4 c = T[4 * (size_t)id + 2];
     const float4 e = T[4 * (size_t)id + 3];
     const float px = d.y * e.z - d.z * e.y, py = d.z * e.x - d.x * e.z, pz = d.x * e.y - d.y * e.x;
     const float det = c.x * px + c.y * py + c.z * pz;
-    if (det == 0.f) return false;
+    if (det == 0.f) return 1;
     const float inv = 1.0f / det;
     const float tx = o.x - a.x, ty = o.y - a.y, tz = o.z - a.z;
     const float u = (tx * px + ty * py + tz * pz) * inv;
-    if (u < 0.f || u > 1.f) return false;
+    if (u < 0.f || u > 1.f) return 1;
     const float qx = ty * c.z - tz * c.y, qy = tz * c.x - tx * c.z, qz = tx * c.y - ty * c.x;
     const float v = (d.x * qx + d.y * qy + d.z * qz) * inv;
-    if (v < 0.f || u + v > 1.f) return false;
+    if (v < 0.f || u + v > 1.f) return 1;
     const float tt = (e.x * qx + e.y * qy + e.z * qz) * inv;
-    if (tt < 0.f) return false;
-    if (ANY ? !(tt <= bound) : !(tt < bound)) return false;
+    if (tt < 0.f) return 1;
+    if (ANY ? !(tt <= bound) : !(tt < bound)) return 1;
     r = tt;
     s = u;
     t = v;
-    return true;
+    return 2;
 }
 
 // Fixes the direction and clips the ray to the root box (kdtree.cpp:519-535).
@@ -211,13 +215,14 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
         const uint32_t id = sc.refs[off + k];
         float rr, ss, tt;
         const float bound = ANY ? tmax_any : T.best_r;
-        const bool hit = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
+        const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
                             : leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt);
         if (COUNT) {
             cnt.refs++;
             cnt.tests++;
+            cnt.full += code >= 1 ? 1u : 0u;
         }
-        if (hit) {
+        if (code == 2) {
             T.best_r = rr;
             T.best_s = ss;
             T.best_t = tt;
