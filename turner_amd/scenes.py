@@ -100,7 +100,7 @@ def cornell_box():
     parts = cornell_walls()
     parts.append(box_mesh((165, 165, 165), (185.5, 0.0, 168.5), -17.0, WHITE))
     parts.append(box_mesh((165, 330, 165), (368.5, 0.0, 351.5), 17.0, WHITE))
-    y = 554.9
+    y = 554.9  # faces down, into the room
     parts.append(_pack(_quad([213, y, 227], [343, y, 227], [343, y, 332], [213, y, 332]),
                        (0.78, 0.78, 0.78)))
     verts, normals, albedo, emission = concat(*parts)
@@ -237,7 +237,7 @@ def unit_cornell_walls():
 def emitter_quad(cx, cz, size, y, radiance):
     """Horizontal emitter facing down (normal -y) at height y."""
     h = size / 2
-    tris = _quad([cx - h, y, cz - h], [cx - h, y, cz + h], [cx + h, y, cz + h], [cx + h, y, cz - h])
+    tris = _quad([cx - h, y, cz - h], [cx + h, y, cz - h], [cx + h, y, cz + h], [cx - h, y, cz + h])
     return _pack(tris, (0.0, 0.0, 0.0), (radiance, radiance, radiance))
 
 
@@ -271,13 +271,14 @@ def occluded_light_cornell(level=8, seed=1):
     parts.append(_mesh_part(v, n, (0.8, 0.8, 0.8)))
     parts.append(emitter_quad(0.5, 0.5, 0.3, 0.999, 50.0))
     h, y = 0.25, 0.9
-    top = _quad([0.5 - h, y, 0.5 - h], [0.5 - h, y, 0.5 + h], [0.5 + h, y, 0.5 + h],
-                [0.5 + h, y, 0.5 - h])   # faces down
-    up = _quad([0.5 - h, y, 0.5 - h], [0.5 + h, y, 0.5 - h], [0.5 + h, y, 0.5 + h],
-               [0.5 - h, y, 0.5 + h])    # faces up (offset by 1e-4 to avoid coplanarity)
+    # two-sided baffle: an up-facing quad 1e-4 above a down-facing one
+    up = _quad([0.5 - h, y, 0.5 - h], [0.5 - h, y, 0.5 + h], [0.5 + h, y, 0.5 + h],
+               [0.5 + h, y, 0.5 - h])
     up = [t + np.array([0, 1e-4, 0], np.float32) for t in up]
-    parts.append(_pack(top, WHITE))
+    down = _quad([0.5 - h, y, 0.5 - h], [0.5 + h, y, 0.5 - h], [0.5 + h, y, 0.5 + h],
+                 [0.5 - h, y, 0.5 + h])
     parts.append(_pack(up, WHITE))
+    parts.append(_pack(down, WHITE))
     verts, normals, albedo, emission = concat(*parts)
     return {"verts": verts, "normals": normals, "albedo": albedo, "emission": emission,
             "camera": camera((0.5, 0.5, -800.0 / 555.0), 39.3), "lights": []}
