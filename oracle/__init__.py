@@ -80,6 +80,10 @@ def lib():
         L.or_ppm.argtypes = [P, C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t]
         L.or_ppm.restype = C.c_size_t
         L.or_introsort_u64.argtypes = [P, C.c_size_t]
+        L.or_flat_inner.argtypes = [C.c_int, C.c_float, C.c_uint32]
+        L.or_flat_inner.restype = C.c_uint64
+        L.or_flat_leaf.argtypes = [C.c_uint32, C.c_uint32]
+        L.or_flat_leaf.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -258,6 +262,26 @@ def introsort_u64(keys):
     k = _c(keys, np.uint64).copy()
     lib().or_introsort_u64(_p(k), len(k))
     return k
+
+
+def flat_inner(axis, split, right):
+    return int(lib().or_flat_inner(axis, split, right))
+
+
+def flat_leaf(a, b=0x3FFFFFFF):
+    return int(lib().or_flat_leaf(a, b))
+
+
+def first_id(node):
+    return int(node) >> 32
+
+
+def has_second(node):
+    return (int(node) & 0xFFFFFFFF) != 0xFFFFFFFF
+
+
+def second_id(node):
+    return (int(node) & 0xFFFFFFFF) >> 2
 
 
 # FlatNode field access (lib/kdtree.h:62-154): low 2 bits 3 = leaf, else axis.

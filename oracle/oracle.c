@@ -611,6 +611,9 @@ static inline uint64_t flat_leaf(uint32_t a, uint32_t b) {
     return ((uint64_t)a << 32) | (uint64_t)((uint32_t)(b << 2) | 3u);
 }
 
+uint64_t or_flat_inner(int axis, float split, uint32_t right) { return flat_inner(axis, split, right); }
+uint64_t or_flat_leaf(uint32_t a, uint32_t b) { return flat_leaf(a, b); }
+
 typedef struct {
     uint64_t* v;
     size_t n, cap;

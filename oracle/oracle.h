@@ -139,6 +139,10 @@ float or_gamma(float light, float inverse_gamma);
 /* Writes PPM text into buf (cap bytes); returns length (excluding NUL). */
 size_t or_ppm(const float* rgba, uint32_t w, uint32_t h, char* buf, size_t cap);
 
+/* FlatNode encoding (lib/kdtree.h:62-154) as the builder writes it. */
+uint64_t or_flat_inner(int axis, float split, uint32_t right);
+uint64_t or_flat_leaf(uint32_t a, uint32_t b);
+
 /* libstdc++ std::sort restated on 32-bit keys (test hook for the event sort). */
 void or_introsort_u64(uint64_t* keys, size_t n);
 

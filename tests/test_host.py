@@ -1,0 +1,108 @@
+"""CPU tests of the product's host side (no GPU calls): the SAH builder
+against the oracle, the tree cache, the PPM writer and the C ABI exports."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import turner_amd as T
+from turner_amd import scenes
+from tests import helpers as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _scenes():
+    rng = np.random.default_rng(7)
+    yield "cornell", scenes.cornell_box()["verts"]
+    yield "icosphere4", scenes.intersect_scene(level=4)["verts"]
+    yield "icosphere6", scenes.intersect_scene(level=6)["verts"]
+    yield "sphere_in_cornell", scenes.sphere_in_cornell(level=4)["verts"]
+    yield "occluded", scenes.occluded_light_cornell(level=3)["verts"]
+    yield "soup", H.random_soup(4000, rng)
+    yield "grid", H.grid_mesh(40)
+    yield "coplanar", H.coplanar_stack(1, rng)
+    yield "plane", H.plane_triangles(2, 500, rng)
+    yield "cube", H.cube12()
+    yield "single", H.tri((0, 0, 1), (0, 1, 1), (1, 0, 1))[None]
+
+
+@pytest.mark.parametrize("name,verts", list(_scenes()))
+def test_builder_matches_oracle(name, verts):
+    """The product builder (C++, std::sort, parallel subtrees) and the oracle
+    (C, restated introsort, serial) build bit-identical trees."""
+    nodes, box, height = T.build_tree(verts)
+    ot = O.Tree(verts)
+    assert np.array_equal(nodes, ot.nodes), name
+    assert np.array_equal(box, ot.box)
+    assert height == ot.height()
+    serial, _, _ = T.build_tree(verts, serial=True)
+    assert np.array_equal(serial, nodes)
+
+
+def test_tree_cache_roundtrip_and_validation(tmp_path):
+    v = scenes.intersect_scene(level=3)["verts"]
+    nodes, box, _ = T.build_tree(v)
+    path = str(tmp_path / "tree.cache")
+    T.tree_cache_save(path, v, nodes, box)
+    n2, b2 = T.tree_cache_load(path, v)
+    assert np.array_equal(n2, nodes) and np.array_equal(b2, box)
+    # other geometry: the reference would silently use the stale cache (main.cpp:147-153)
+    v2 = v.copy()
+    v2[0, 0] += 1e-3
+    with pytest.raises(T.TurnerError, match="stale"):
+        T.tree_cache_load(path, v2)
+    with pytest.raises(T.TurnerError):
+        T.tree_cache_load(str(tmp_path / "missing"), v)
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[: len(raw) - 16])
+    with pytest.raises(T.TurnerError, match="truncated"):
+        T.tree_cache_load(path, v)
+    bad = bytearray(raw)
+    hdr = 4 + 4 + 4 + 8 + 8 + 24
+    bad[hdr:hdr + 8] = (0xFFFFFFF0).to_bytes(8, "little")  # root: inner node, right out of range
+    open(path, "wb").write(bytes(bad))
+    with pytest.raises(T.TurnerError, match="corrupt"):
+        T.tree_cache_load(path, v)
+
+
+def test_ppm_writer_matches_reference_format():
+    """lib/raster.h:79-100 and tests/test_raster.cpp:44-57"""
+    img = np.zeros((2, 2, 4), np.float32)
+    img[1, 0] = 1
+    assert T.write_ppm(img) == "P3\n2 2\n255\n  0   0   0   0   0   0\n255 255 255   0   0   0"
+    rng = np.random.default_rng(0)
+    img = rng.uniform(-0.2, 1.3, (7, 5, 4)).astype(np.float32)
+    assert T.write_ppm(img) == O.ppm(img)
+
+
+def test_every_declared_symbol_is_exported():
+    header = open(os.path.join(ROOT, "include", "turner_amd.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|long|void|const char\*)\s+(tn_\w+)\(", header, re.M))
+    assert len(declared) >= 20
+    lib = C.CDLL(os.path.join(ROOT, "turner_amd", "libturner_amd.so"))
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    from turner_amd import _lib
+    assert declared <= set(_lib.SIGNATURES)
+
+
+def test_invalid_arguments_fail_loudly():
+    with pytest.raises(T.TurnerError):
+        T.build_tree(np.zeros((0, 9), np.float32))
+    L = T.lib()
+    assert L.tn_scene_destroy(None) == T._lib.TN_ERR_INVALID
+    assert L.tn_intersect(None, None, None, 0, None, None, None, None, 0, None) == T._lib.TN_ERR_INVALID
+    assert b"NULL" in L.tn_last_error() or b"scene" in L.tn_last_error()
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(T.TurnerError) as e:
+        T.KDTree(H.cube12())
+    assert e.value.code == T._lib.TN_ERR_NODEVICE

@@ -471,37 +471,43 @@ struct RenderBuffers {
 
 std::mutex g_render_mutex;
 
-int alloc_queues(RenderBuffers& rb, uint32_t cap) {
-    const size_t path = (size_t)cap * (16 * 4) * 2 + (size_t)cap * 16;
-    const size_t shadow = (size_t)2 * cap * 48;
-    const size_t ctr = 4 * kMaxLevels * 4 + 64 + 2 * 8;
-    const size_t need = path + shadow + ctr + 256;
-    if (rb.base && rb.bytes >= need && rb.q.cap >= cap) return TN_OK;
-    if (rb.base) (void)hipFree(rb.base);
-    rb.base = nullptr;
-    TN_HIP(hipMalloc(&rb.base, need));
-    rb.bytes = need;
-    char* p = rb.base;
+// Carves the queue buffers out of one allocation; with base == nullptr it
+// only measures (every piece is rounded up to 256 bytes).
+size_t carve(Queues& q, char* base, uint32_t cap) {
+    size_t off = 0;
     auto take = [&](size_t b) {
-        char* r = p;
-        p += (b + 255) & ~(size_t)255;
+        char* r = base ? base + off : nullptr;
+        off += (b + 255) & ~(size_t)255;
         return r;
     };
     for (int k = 0; k < 2; ++k) {
-        rb.q.o[k] = (float4*)take((size_t)cap * 16);
-        rb.q.d[k] = (float4*)take((size_t)cap * 16);
-        rb.q.w[k] = (float4*)take((size_t)cap * 16);
-        rb.q.rng[k] = (uint4*)take((size_t)cap * 16);
+        q.o[k] = (float4*)take((size_t)cap * 16);
+        q.d[k] = (float4*)take((size_t)cap * 16);
+        q.w[k] = (float4*)take((size_t)cap * 16);
+        q.rng[k] = (uint4*)take((size_t)cap * 16);
     }
-    rb.q.hit = (uint4*)take((size_t)cap * 16);
-    rb.q.scap = 2 * cap;
-    rb.q.so = (float4*)take((size_t)rb.q.scap * 16);
-    rb.q.sd = (float4*)take((size_t)rb.q.scap * 16);
-    rb.q.sc = (float4*)take((size_t)rb.q.scap * 16);
-    rb.q.cnt = (uint32_t*)take(4 * kMaxLevels * 4 + 64);
-    rb.q.totals = (unsigned long long*)take(16);
-    rb.q.cap = cap;
-    return TN_OK;
+    q.hit = (uint4*)take((size_t)cap * 16);
+    q.scap = 2 * cap;
+    q.so = (float4*)take((size_t)q.scap * 16);
+    q.sd = (float4*)take((size_t)q.scap * 16);
+    q.sc = (float4*)take((size_t)q.scap * 16);
+    q.cnt = (uint32_t*)take((4 * kMaxLevels + 1) * sizeof(uint32_t));
+    q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
+    q.cap = cap;
+    return off;
+}
+
+int alloc_queues(RenderBuffers& rb, uint32_t cap) {
+    Queues probe{};
+    const size_t need = carve(probe, nullptr, cap);
+    if (rb.base && rb.bytes >= need && rb.q.cap >= cap) return TN_OK;
+    if (rb.base) (void)hipFree(rb.base);
+    rb.base = nullptr;
+    rb.bytes = 0;
+    TN_HIP(hipMalloc(&rb.base, need));
+    rb.bytes = need;
+    const size_t used = carve(rb.q, rb.base, cap);
+    return used <= need ? TN_OK : set_error(TN_ERR_INVALID, "queue layout exceeds its allocation");
 }
 
 }  // namespace
