@@ -160,7 +160,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes,
-            "kernel": "k_rays<closest hit> (tn_intersect)",
+            "kernel": "k_rays_coop<closest hit> (tn_intersect)",
         },
         "wall_s_timed_region": round(wall, 4),
     }

@@ -99,6 +99,14 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
 #define TN_ISECT_EXHAUSTIVE 1u /* visit every leaf the ray crosses (reference order) */
 #define TN_ISECT_MT 2u         /* FP32 Moeller-Trumbore leaf test instead of the reference's */
 #define TN_ISECT_COUNT 4u      /* accumulate work counters (tn_get_counters) */
+/* Traversal schedules (default: wave-cooperative leaf tests -- the wave pools
+ * the triangle references of all its lanes' current leaves and tests them 64
+ * at a time).  Results are identical under every schedule. */
+#define TN_ISECT_TICK 8u       /* each lane: a few node steps or one triangle test per step */
+#define TN_ISECT_LEAF 16u      /* each lane: one leaf (all of its triangles) per step */
+/* Bits 24..27: occupancy variant of the default closest-hit kernel (waves
+ * per SIMD 5..8, register budget accordingly; 0 = the tuned default). */
+#define TN_ISECT_WAVES(w) ((uint32_t)(w) << 24)
 
 /*
  * Batched twin of KDTreeIntersection::intersect(ray, r, a, b) (lib/kdtree.h:291,

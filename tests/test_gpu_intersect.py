@@ -11,7 +11,8 @@ import pytest
 import oracle as O
 import turner_amd as T
 from turner_amd import scenes
-from turner_amd._lib import TN_ISECT_COUNT, TN_ISECT_EXHAUSTIVE, TN_ISECT_MT
+from turner_amd._lib import (TN_ISECT_COUNT, TN_ISECT_EXHAUSTIVE, TN_ISECT_LEAF, TN_ISECT_MT,
+                             TN_ISECT_TICK)
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
@@ -86,8 +87,9 @@ def test_icosphere_bit_exact(gpu, level):
     d = np.concatenate([d, hd])
     print(f"[icosphere level {level}] {tree.num_triangles()} tris, {len(o)} rays", flush=True)
     ref = oracle_hits(tree, o, d)
-    assert_bit_exact(gpu_hits(tree, o, d), ref, "early-exit")
-    assert_bit_exact(gpu_hits(tree, o, d, TN_ISECT_EXHAUSTIVE), ref, "exhaustive")
+    for sched, name in ((0, "cooperative"), (TN_ISECT_LEAF, "leaf"), (TN_ISECT_TICK, "tick")):
+        assert_bit_exact(gpu_hits(tree, o, d, sched), ref, name + " early-exit")
+        assert_bit_exact(gpu_hits(tree, o, d, sched | TN_ISECT_EXHAUSTIVE), ref, name + " exhaustive")
 
 
 def test_soup_and_grid_ties(gpu):
@@ -104,7 +106,8 @@ def test_soup_and_grid_ties(gpu):
             o = np.concatenate([o, np.c_[k, np.full(4096, 5, np.float32)]])
             d = np.concatenate([d, np.tile(np.array([[0, 0, -1]], np.float32), (4096, 1))])
         ref = oracle_hits(tree, o, d)
-        assert_bit_exact(gpu_hits(tree, o, d), ref, "ties")
+        for sched in (0, TN_ISECT_LEAF, TN_ISECT_TICK):
+            assert_bit_exact(gpu_hits(tree, o, d, sched), ref, f"ties sched {sched}")
 
 
 def _cos_incidence(verts, ids, d):

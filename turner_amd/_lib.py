@@ -23,6 +23,8 @@ TN_SCENE_SERIAL_BUILD = 1
 TN_ISECT_EXHAUSTIVE = 1
 TN_ISECT_MT = 2
 TN_ISECT_COUNT = 4
+TN_ISECT_TICK = 8
+TN_ISECT_LEAF = 16
 TN_RENDER_COSINE = 1
 TN_RENDER_AREA_LIGHTS = 2
 TN_RENDER_REF_JITTER = 4

@@ -38,7 +38,7 @@ namespace {
 // and, after every leaf step, hands lanes whose ray finished (or missed the
 // root box) the chunk's next rays (ballot + mbcnt), so a wave is not held by
 // its slowest ray.  ANY = visibility query against tmax (tn_occluded).
-template <bool EXH, bool MT, bool ANY, bool COUNT>
+template <int SCHED, bool EXH, bool MT, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __restrict__ o3,
                                                  const float* __restrict__ d3,
                                                  const float* __restrict__ tmax, uint32_t n,
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
             continue;
         }
         if (ray != kNone) {
-            if (trav_step<kStack, EXH, MT, ANY, COUNT>(T, sc, tmx, my, kBlock, cnt)) {
+            if (trav_advance<SCHED, kStack, EXH, MT, ANY, COUNT>(T, sc, tmx, my, kBlock, cnt)) {
                 const bool hit = T.best_id != kInvalidId;
                 if (ANY) {
                     occ[ray] = hit ? 1 : 0;
@@ -109,6 +109,69 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
             }
         }
     }
+    if (cnt.incomplete) atomicAdd(status, 1u);
+    if (COUNT) {
+        atomicAdd(&red[0], cnt.inner);
+        atomicAdd(&red[1], cnt.leaves);
+        atomicAdd(&red[2], cnt.refs);
+        atomicAdd(&red[3], cnt.tests);
+        atomicAdd(&red[4], nrays);
+        atomicAdd(&red[5], cnt.restarts);
+        atomicAdd(&red[6], cnt.full);
+        __syncthreads();
+        if (threadIdx.x < 7) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+    }
+}
+
+// Wave-cooperative variant of k_rays (coop_loop): same chunking and XCD
+// mapping, pooled leaf tests.  WPE caps the VGPRs so that WPE waves fit on
+// each SIMD.
+constexpr int kCoopWavesPerEU = 7;  // measured best (5: 964, 6: 964, 7: 1036, 8: 942 Mrays/s)
+template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rays_coop(
+    SceneView sc, const float* __restrict__ o3, const float* __restrict__ d3,
+    const float* __restrict__ tmax, uint32_t n, uint32_t* __restrict__ out_id,
+    float* __restrict__ out_r, float* __restrict__ out_s, float* __restrict__ out_t,
+    uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status) {
+    __shared__ uint2 stk[kStack * kBlock];
+    __shared__ unsigned long long slots[kBlock];
+    __shared__ uint32_t red[7];
+    if (COUNT && threadIdx.x < 7) red[threadIdx.x] = 0;
+    if (COUNT) __syncthreads();
+    const uint32_t per_xcd = gridDim.x / kXcds;
+    const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
+    const uint32_t first = wave * (uint32_t)kChunk;
+    bool given = false;
+    uint32_t nrays = 0;
+    TraversalCounters cnt;
+    coop_loop<kStack, EXH, MT, ANY, COUNT>(
+        sc, stk + threadIdx.x, kBlock, slots + (threadIdx.x & ~63u),
+        [&](uint32_t& b, uint32_t& e) {
+            if (given || first >= n) return false;
+            given = true;
+            b = first;
+            e = min(first + (uint32_t)kChunk, n);
+            return true;
+        },
+        [&](uint32_t k, float3& o, float3& d, float& tm) {
+            o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
+            d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
+            if (ANY) tm = tmax[k];
+            if (COUNT) nrays++;
+        },
+        [&](uint32_t k, const Trav& T, bool traversed) {
+            const bool hit = traversed && T.best_id != kInvalidId;
+            if (ANY) {
+                occ[k] = hit ? 1 : 0;
+            } else {
+                out_id[k] = hit ? T.best_id : kInvalidId;
+                out_r[k] = hit ? T.best_r : FLT_MAX;
+                out_s[k] = hit ? T.best_s : 0.f;
+                out_t[k] = hit ? T.best_t : 0.f;
+            }
+        },
+        cnt);
     if (cnt.incomplete) atomicAdd(status, 1u);
     if (COUNT) {
         atomicAdd(&red[0], cnt.inner);
@@ -265,18 +328,44 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
     const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
-               cnt = flags & TN_ISECT_COUNT, any = occ != nullptr;
+               cnt = flags & TN_ISECT_COUNT, any = occ != nullptr,
+               tick = flags & TN_ISECT_TICK, leaf = flags & TN_ISECT_LEAF;
     auto go = [&](auto kern) {
         kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, id, r, s, t, occ, sc->d_counters,
                                          sc->d_status);
     };
-#define TN_SEL(A, C)                                                     \
+    const uint32_t wpe = (flags >> 24) & 15u;  // occupancy variant (tuning), 0 = default
+    if (!any && !cnt && !tick && !(flags & TN_ISECT_LEAF) && !exh && !mt && wpe) {
+        if (wpe == 5) go(k_rays_coop<false, false, false, false, 5>);
+        else if (wpe == 7) go(k_rays_coop<false, false, false, false, 7>);
+        else if (wpe == 8) go(k_rays_coop<false, false, false, false, 8>);
+        else go(k_rays_coop<false, false, false, false, 6>);
+        TN_HIP(hipGetLastError());
+        return TN_OK;
+    }
+#define TN_SEL2(SC, A, C)                                                \
     if (exh) {                                                           \
-        if (mt) go(k_rays<true, true, A, C>);                            \
-        else go(k_rays<true, false, A, C>);                              \
+        if (mt) go(k_rays<SC, true, true, A, C>);                        \
+        else go(k_rays<SC, true, false, A, C>);                          \
     } else {                                                             \
-        if (mt) go(k_rays<false, true, A, C>);                           \
-        else go(k_rays<false, false, A, C>);                             \
+        if (mt) go(k_rays<SC, false, true, A, C>);                       \
+        else go(k_rays<SC, false, false, A, C>);                         \
+    }
+#define TN_SELC(A, C)                                                    \
+    if (exh) {                                                           \
+        if (mt) go(k_rays_coop<true, true, A, C>);                       \
+        else go(k_rays_coop<true, false, A, C>);                         \
+    } else {                                                             \
+        if (mt) go(k_rays_coop<false, true, A, C>);                      \
+        else go(k_rays_coop<false, false, A, C>);                        \
+    }
+#define TN_SEL(A, C)                                                     \
+    if (tick) {                                                          \
+        TN_SEL2(kSchedTick, A, C)                                        \
+    } else if (leaf) {                                                   \
+        TN_SEL2(kSchedLeaf, A, C)                                        \
+    } else {                                                             \
+        TN_SELC(A, C)                                                    \
     }
     if (any) {
         TN_SEL(true, false)
@@ -285,6 +374,8 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     } else {
         TN_SEL(false, false)
     }
+#undef TN_SEL2
+#undef TN_SELC
 #undef TN_SEL
     TN_HIP(hipGetLastError());
     return TN_OK;

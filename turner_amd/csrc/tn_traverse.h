@@ -25,8 +25,10 @@
 //   * near-first early exit: traversal ends once the best hit lies before the
 //     current leaf's exit by a 2^-16 relative margin (no later leaf can then
 //     hold a strictly closer hit).  TN_ISECT_EXHAUSTIVE disables it;
-//   * the traversal is a per-lane state machine (trav_init / trav_step, one
-//     leaf per step) so a wave can hand finished lanes new rays.
+//   * the traversal is a per-lane state machine (trav_init / trav_tick: a
+//     few node steps or one triangle test per tick) so lanes in different
+//     phases share a wave's instructions and a wave can hand finished lanes
+//     new rays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,9 +58,10 @@ struct TraversalCounters {
 };
 
 struct Trav {
-    float3 o, d, fd, dinv;
+    float3 o, d, dinv;
     float tenter, texit, root_exit;
     uint32_t node;
+    uint32_t leaf_off, leaf_left;  // triangle references still to test in the current leaf
     int head, count, restarts;
     bool lost;
     uint32_t best_id;
@@ -91,6 +94,13 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint3
     if (den == 0.f) return 0;  // r = lowest() < 0
     const float nom = b.x * (a.x - o.x) + b.y * (a.y - o.y) + b.z * (a.z - o.z);
     if (certainly_negative(nom, den)) return 0;
+    if (!ANY) {
+        // r = nom / den certainly >= bound (it cannot beat the current best):
+        // compare against bound * den widened by 2^-20, far beyond the
+        // rounding of the division and of the product.
+        const float q = bound * den * (1.0f + 0x1p-20f);
+        if (fabsf(q) >= 0x1p-100f && (den > 0.f ? nom >= q : nom <= q)) return 0;
+    }
     const float rr = nom / den;
     if (rr < 0) return 0;
     if (ANY ? !(rr <= bound) : !(rr < bound)) return 0;
@@ -101,11 +111,20 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint3
     const float wv = wx * e.x + wy * e.y + wz * e.z;
     const float wu = wx * c.x + wy * c.y + wz * c.z;
     const float snum = a.w * wv - b.w * wu;
-    if (certainly_negative(snum, e.w)) return 1;
+    const float tnum = a.w * wu - c.w * wv;
+    // The reference rejects on s < 0, t < 0 or 1 < s + t; the verdict does
+    // not depend on the order of the checks, so decide what the numerators
+    // decide before dividing: a negative quotient, or s + t above 1 by a
+    // 2^-18 margin (s and t then have the sign of the denominator, so the
+    // sum of the numerators carries only rounding error).
+    if (certainly_negative(snum, e.w) || certainly_negative(tnum, e.w)) return 1;
+    {
+        const float sum = snum + tnum;
+        const float q = e.w * (1.0f + 0x1p-18f);
+        if (fabsf(q) >= 0x1p-100f && (e.w < 0.f ? sum < q : sum > q)) return 1;
+    }
     const float ss = snum / e.w;
     if (ss < 0) return 1;
-    const float tnum = a.w * wu - c.w * wv;
-    if (certainly_negative(tnum, e.w)) return 1;
     const float tt = tnum / e.w;
     if (tt < 0 || 1 < ss + tt) return 1;
     r = rr;
@@ -150,7 +169,6 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     if (fd.x == 0.f) fd.x = kDirEps;
     if (fd.y == 0.f) fd.y = kDirEps;
     if (fd.z == 0.f) fd.z = kDirEps;
-    T.fd = fd;
     T.dinv = make_float3(1 / fd.x, 1 / fd.y, 1 / fd.z);
     const float3 di = T.dinv;
     const float tx1 = (sc.bmin[0] - o.x) * di.x, tx2 = (sc.bmax[0] - o.x) * di.x;
@@ -165,6 +183,8 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     T.texit = hi;
     T.root_exit = hi;
     T.node = 0;
+    T.leaf_off = 0;
+    T.leaf_left = 0;
     T.head = 0;
     T.count = 0;
     T.restarts = 0;
@@ -176,8 +196,79 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     return !(hi < lo);
 }
 
-// Descends from T.node to the next leaf, tests it and moves to the next
-// stack entry.  Returns true when the traversal is complete.
+// Descends from T.node to the next leaf (T.texit becomes the leaf's exit).
+// Returns the leaf node.
+template <int S, bool COUNT>
+__device__ __forceinline__ uint2 trav_descend(Trav& T, const SceneView& sc, uint2* stk, int stride,
+                                              TraversalCounters& cnt) {
+    uint32_t node = T.node;
+    const float tenter = T.tenter;
+    float texit = T.texit;
+    uint2 nd = sc.nodes[node];
+    while ((nd.y & 3u) != 3u) {
+        if (COUNT) cnt.inner++;
+        const uint32_t ax = nd.y & 3u;
+        const float tt = (__uint_as_float(nd.x) - pick(T.o, ax)) * pick(T.dinv, ax);
+        uint32_t nearn = node + 1, farn = nd.y >> 2;
+        if (pick(T.d, ax) < 0.f) {  // fixed direction <= 0 (zeros become +1e-5)
+            const uint32_t tmp = nearn;
+            nearn = farn;
+            farn = tmp;
+        }
+        if (texit < tt) {
+            node = nearn;
+        } else if (tt < tenter) {
+            node = farn;
+        } else {
+            stk[T.head * stride] = make_uint2(farn, __float_as_uint(texit));
+            T.head = (T.head + 1) & (S - 1);
+            if (T.count < S) ++T.count;
+            else T.lost = true;
+            node = nearn;
+            texit = tt;
+        }
+        nd = sc.nodes[node];
+    }
+    if (COUNT) cnt.leaves++;
+    T.node = node;
+    T.texit = texit;
+    return nd;
+}
+
+// After a leaf: early exit, next stack entry or restart.  Returns true when
+// the traversal is complete.
+template <int S, bool EXH, bool ANY>
+__device__ __forceinline__ bool trav_after_leaf(Trav& T, float tmax_any, const uint2* stk,
+                                                int stride, TraversalCounters& cnt) {
+    const float texit = T.texit;
+    if (!EXH) {
+        if (!ANY && T.best_id != kInvalidId && T.best_r * kExitMargin < texit) return true;
+        if (ANY && tmax_any * kExitMargin < texit) return true;
+    }
+    if (T.count == 0) {
+        if (!T.lost || !(texit < T.root_exit)) return true;
+        if (++T.restarts > kMaxRestarts) {
+            cnt.incomplete = true;
+            return true;
+        }
+        T.node = 0;
+        T.tenter = texit;
+        T.texit = T.root_exit;
+        T.lost = false;
+        return false;
+    }
+    T.head = (T.head - 1) & (S - 1);
+    --T.count;
+    const uint2 e = stk[T.head * stride];
+    T.node = e.x;
+    T.tenter = texit;
+    T.texit = __uint_as_float(e.y);
+    return false;
+}
+
+// One leaf per step: descend to the next leaf, test all of its triangles,
+// then the early-exit check and the move to the next stack entry.  Returns
+// true when the traversal is complete.
 template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
 __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tmax_any, uint2* stk,
                                           int stride, TraversalCounters& cnt) {
@@ -190,7 +281,7 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
         const uint32_t ax = nd.y & 3u;
         const float tt = (__uint_as_float(nd.x) - pick(T.o, ax)) * pick(T.dinv, ax);
         uint32_t nearn = node + 1, farn = nd.y >> 2;
-        if (pick(T.fd, ax) <= 0.f) {
+        if (pick(T.d, ax) < 0.f) {  // fixed direction <= 0 (zeros become +1e-5)
             const uint32_t tmp = nearn;
             nearn = farn;
             farn = tmp;
@@ -255,6 +346,122 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
     return false;
 }
 
+// Traversal schedules: one leaf per step (a wave runs its largest leaf) or
+// ticks of a few node steps / one triangle test.
+enum Schedule { kSchedLeaf = 0, kSchedTick = 1, kSchedCoop = 2 };
+
+template <int SCHED, int S, bool EXH, bool MT, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_advance(Trav& T, const SceneView& sc, float tmax_any,
+                                             uint2* stk, int stride, TraversalCounters& cnt);
+
+constexpr int kDescendPerTick = 8;
+
+// One tick of the traversal state machine: at most kDescendPerTick node
+// steps towards the next leaf, then at most one triangle test, then -- when
+// the leaf is exhausted -- the early-exit check and the move to the next
+// stack entry.  Returns true when the traversal is complete.  Lanes of a
+// wave advance through different phases in the same tick, so a wave never
+// waits for the largest leaf among its lanes.
+template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_tick(Trav& T, const SceneView& sc, float tmax_any, uint2* stk,
+                                          int stride, TraversalCounters& cnt) {
+    static_assert((S & (S - 1)) == 0, "stack size must be a power of two");
+    if (T.leaf_left == 0) {
+        uint32_t node = T.node;
+        float tenter = T.tenter, texit = T.texit;
+        uint2 nd = sc.nodes[node];
+        int steps = 0;
+        while ((nd.y & 3u) != 3u && steps < kDescendPerTick) {
+            if (COUNT) cnt.inner++;
+            const uint32_t ax = nd.y & 3u;
+            const float tt = (__uint_as_float(nd.x) - pick(T.o, ax)) * pick(T.dinv, ax);
+            uint32_t nearn = node + 1, farn = nd.y >> 2;
+            if (pick(T.d, ax) < 0.f) {  // fixed direction <= 0 (zeros become +1e-5)
+                const uint32_t tmp = nearn;
+                nearn = farn;
+                farn = tmp;
+            }
+            if (texit < tt) {
+                node = nearn;
+            } else if (tt < tenter) {
+                node = farn;
+            } else {
+                stk[T.head * stride] = make_uint2(farn, __float_as_uint(texit));
+                T.head = (T.head + 1) & (S - 1);
+                if (T.count < S) ++T.count;
+                else T.lost = true;
+                node = nearn;
+                texit = tt;
+            }
+            nd = sc.nodes[node];
+            ++steps;
+        }
+        T.node = node;
+        T.tenter = tenter;
+        T.texit = texit;
+        if ((nd.y & 3u) != 3u) return false;  // still descending
+        if (COUNT) cnt.leaves++;
+        T.leaf_off = nd.x;
+        T.leaf_left = nd.y >> 2;
+        if (T.leaf_left == 0) goto leaf_done;
+    }
+    {
+        const uint32_t id = sc.refs[T.leaf_off];
+        ++T.leaf_off;
+        --T.leaf_left;
+        float rr, ss, tt;
+        const float bound = ANY ? tmax_any : T.best_r;
+        const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
+                            : leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt);
+        if (COUNT) {
+            cnt.refs++;
+            cnt.tests++;
+            cnt.full += code >= 1 ? 1u : 0u;
+        }
+        if (code == 2) {
+            T.best_r = rr;
+            T.best_s = ss;
+            T.best_t = tt;
+            T.best_id = id;
+            if (ANY) return true;
+        }
+        if (T.leaf_left != 0) return false;
+    }
+leaf_done : {
+    const float texit = T.texit;
+    if (!EXH) {
+        if (!ANY && T.best_id != kInvalidId && T.best_r * kExitMargin < texit) return true;
+        if (ANY && tmax_any * kExitMargin < texit) return true;
+    }
+    if (T.count == 0) {
+        if (!T.lost || !(texit < T.root_exit)) return true;
+        if (++T.restarts > kMaxRestarts) {
+            cnt.incomplete = true;
+            return true;
+        }
+        T.node = 0;
+        T.tenter = texit;
+        T.texit = T.root_exit;
+        T.lost = false;
+        return false;
+    }
+    T.head = (T.head - 1) & (S - 1);
+    --T.count;
+    const uint2 e = stk[T.head * stride];
+    T.node = e.x;
+    T.tenter = texit;
+    T.texit = __uint_as_float(e.y);
+    return false;
+}
+}
+
+template <int SCHED, int S, bool EXH, bool MT, bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_advance(Trav& T, const SceneView& sc, float tmax_any,
+                                             uint2* stk, int stride, TraversalCounters& cnt) {
+    if (SCHED == kSchedTick) return trav_tick<S, EXH, MT, ANY, COUNT>(T, sc, tmax_any, stk, stride, cnt);
+    return trav_step<S, EXH, MT, ANY, COUNT>(T, sc, tmax_any, stk, stride, cnt);
+}
+
 // Whole traversal of one ray (closest hit, or visibility up to tmax_any).
 template <int S, bool EXH, bool MT, bool ANY, bool COUNT>
 __device__ __forceinline__ uint32_t traverse(const SceneView& sc, float3 o, float3 d, float tmax_any,
@@ -275,6 +482,126 @@ __device__ __forceinline__ uint32_t traverse(const SceneView& sc, float3 o, floa
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+
+// Wave-cooperative traversal loop.  Every round, each lane with a ray
+// descends to its next leaf; then the wave pools the (ray, triangle
+// reference) pairs of all its lanes' leaves and tests them 64 at a time --
+// lane j takes pair base + j, finds the owning lane by binary search over the
+// exclusive prefix of leaf sizes and fetches the owner's ray with
+// ds_bpermute -- so a round costs ceil(sum of leaf sizes / 64) tests instead
+// of the largest leaf.  A hit is folded into the owner's 64-bit LDS slot with
+// atomicMin on (r, position in leaf): the minimum is the first of the leaf's
+// closest hits in leaf order, i.e. the reference's strict-< scan
+// (kdtree.cpp:587-605); the owner re-tests that triangle to recover r, s, t.
+// Finished lanes take the next rays of the wave's chunk.
+//   chunk(begin, end): called by all lanes of the wave; returns false when no
+//                      work is left (may update begin/end);
+//   load(k, o, d, tmax), store(k, T, traversed): per-ray I/O.
+template <int S, bool EXH, bool MT, bool ANY, bool COUNT, class Chunk, class Load, class Store>
+__device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int stride,
+                                          unsigned long long* wslot, Chunk chunk, Load load,
+                                          Store store, TraversalCounters& cnt) {
+    const uint32_t lane = threadIdx.x & 63u;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t next = 0, end = 0, ray = kNone;
+    bool more = true;
+    Trav T;
+    float tmx = 0.f;
+    for (;;) {
+        const uint64_t idle = __ballot(ray == kNone);
+        if (idle && next >= end && more) more = chunk(next, end);
+        if (idle && next < end) {
+            const uint32_t k = next + lanes_below(idle);
+            next += (uint32_t)__popcll(idle);
+            if (ray == kNone && k < end) {
+                float3 o, d;
+                load(k, o, d, tmx);
+                if (trav_init(T, sc, o, d)) ray = k;
+                else store(k, T, false);
+            }
+        }
+        if (__ballot(ray != kNone) == 0) {
+            if (!more && next >= end) break;
+            continue;
+        }
+        // phase 1: every active lane descends to its next leaf
+        uint32_t loff = 0, lcnt = 0;
+        if (ray != kNone) {
+            const uint2 nd = trav_descend<S, COUNT>(T, sc, stk, stride, cnt);
+            loff = nd.x;
+            lcnt = nd.y >> 2;
+        }
+        // phase 2: pooled leaf tests
+        uint32_t incl = lcnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += v;
+        }
+        const uint32_t excl = incl - lcnt;
+        const uint32_t total = __shfl(incl, 63, 64);
+        const float bound_own = ANY ? tmx : T.best_r;
+        wslot[lane] = ~0ull;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t item = base + lane;
+            uint32_t own = 0;
+#pragma unroll
+            for (uint32_t step = 32; step >= 1; step >>= 1) {
+                const uint32_t cand = own + step;
+                const uint32_t pc = __shfl(excl, (int)(cand & 63u), 64);
+                if (cand < 64u && pc <= item) own = cand;
+            }
+            const uint32_t loc = item - __shfl(excl, (int)own, 64);
+            const uint32_t off = __shfl(loff, (int)own, 64);
+            const float3 ro = make_float3(__shfl(T.o.x, (int)own, 64), __shfl(T.o.y, (int)own, 64),
+                                          __shfl(T.o.z, (int)own, 64));
+            const float3 rd = make_float3(__shfl(T.d.x, (int)own, 64), __shfl(T.d.y, (int)own, 64),
+                                          __shfl(T.d.z, (int)own, 64));
+            const float bnd = __shfl(bound_own, (int)own, 64);
+            if (item < total) {
+                const uint32_t id = sc.refs[off + loc];
+                float rr, ss, tt;
+                const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt)
+                                    : leaf_test_ref<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt);
+                if (COUNT) {
+                    cnt.refs++;
+                    cnt.tests++;
+                    cnt.full += code >= 1 ? 1u : 0u;
+                }
+                if (code == 2) {
+                    // r >= 0 here; -0 maps to +0 so the key orders like the reference's <
+                    const uint32_t rb = __float_as_uint(rr + 0.0f);
+                    atomicMin(&wslot[own], ((unsigned long long)rb << 32) | loc);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const unsigned long long key = wslot[lane];
+        if (lcnt) {
+            if (key != ~0ull) {
+                const uint32_t id = sc.refs[loff + (uint32_t)key];
+                float rr, ss, tt;
+                if (MT) leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
+                else leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
+                T.best_r = rr;
+                T.best_s = ss;
+                T.best_t = tt;
+                T.best_id = id;
+                if (ANY) {
+                    store(ray, T, true);
+                    ray = kNone;
+                }
+            }
+            if (ray != kNone && trav_after_leaf<S, EXH, ANY>(T, tmx, stk, stride, cnt)) {
+                if (COUNT) cnt.restarts += T.restarts;
+                store(ray, T, true);
+                ray = kNone;
+            }
+        }
+    }
 }
 
 }  // namespace tn
