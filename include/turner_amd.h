@@ -196,6 +196,19 @@ int tn_render_accumulate(tn_scene* scene, const tn_camera* cam, const tn_light* 
                          uint32_t num_lights, const tn_render_params* params, float* accum,
                          tn_render_stats* stats, void* stream);
 
+/*
+ * Batched twin of trace(ray, tree_intersection, lights, depth, conf)
+ * (pathtracer.cpp:14-102, trace.h:18-20): traces n caller rays (device
+ * arrays o3, d3) starting at recursion level `depth` and ADDS each ray's
+ * radiance (RGBA) to rgba[4 i .. 4 i + 3] (device).  The estimator, flags,
+ * background, m (mc_samples) and max_depth come from params (width, height,
+ * pixel_samples and jitter are ignored).  depth > max_depth traces nothing
+ * (the reference returns Color{}).  Asynchronous on `stream`.
+ */
+int tn_trace(tn_scene* scene, const float* o3, const float* d3, uint32_t n, int32_t depth,
+             const tn_light* lights, uint32_t num_lights, const tn_render_params* params,
+             float* rgba, tn_render_stats* stats, void* stream);
+
 /* Tone map (lib/effects.h:15-49) of device RGBA sums: out = gamma(exposure(sum/spp)). */
 int tn_tonemap(const float* accum, float* out, uint32_t num_pixels, float inv_spp,
                float exposure, float inverse_gamma, int gamma_enabled, void* stream);

@@ -89,6 +89,8 @@ SIGNATURES = {
                             C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats)]),
     "tn_render_accumulate": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(Light), _U32,
                                        C.POINTER(RenderParams), _P, C.POINTER(RenderStats), _P]),
+    "tn_trace": (C.c_int, [_P, _P, _P, _U32, C.c_int32, C.POINTER(Light), _U32,
+                           C.POINTER(RenderParams), _P, C.POINTER(RenderStats), _P]),
     "tn_tonemap": (C.c_int, [_P, _P, _U32, C.c_float, C.c_float, C.c_float, C.c_int, _P]),
     "tn_write_ppm": (C.c_long, [_P, _U32, _U32, _P, C.c_size_t]),
     "tn_kdtree_build": (C.c_int, [_P, _U32, _U32, C.POINTER(C.POINTER(C.c_uint64)),

@@ -79,8 +79,10 @@ def build_cli():
     srcs = [os.path.join(host, f) for f in ("pathtracer_main.cpp", "turner.cpp")]
     if not all(os.path.exists(s) for s in srcs):
         return None
-    _run(["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), "-I", host,
-          *srcs, "-o", out, f"-L{HERE}", "-lturner_amd", f"-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-O2", "-std=c++17", "-pthread", "-D__HIP_PLATFORM_AMD__",
+          "-I", os.path.join(ROOT, "include"), "-I", host, "-I", os.path.join(ROCM, "include"),
+          *srcs, "-o", out, f"-L{HERE}", "-lturner_amd", f"-L{os.path.join(ROCM, 'lib')}",
+          "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"])
     return out
 
 

@@ -180,80 +180,70 @@ __global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q, uint
     Q.rng[0][i] = rng;
 }
 
-// Dynamic-chunk lane-refill traversal over a device-counted queue.
-template <bool EXH, bool ANY, class Load, class Store>
-__device__ __forceinline__ void refill_traverse(const SceneView& sc, uint32_t n, uint32_t* work,
-                                                uint2* stk, Load load, Store store) {
-    const uint32_t lane = threadIdx.x & 63u;
-    constexpr uint32_t kNone = 0xFFFFFFFFu;
-    uint32_t next = 0, end = 0, ray = kNone;
-    bool more = true;
-    Trav T;
-    float tmx = 0.f;
-    TraversalCounters cnt;
-    for (;;) {
-        const uint64_t idle = __ballot(ray == kNone);
-        if (idle && next >= end && more) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work, (uint32_t)kChunk);
-            base = __shfl(base, 0);
-            if (base < n) {
-                next = base;
-                end = min(base + (uint32_t)kChunk, n);
-            } else {
-                more = false;
-            }
-        }
-        if (idle && next < end) {
-            const uint32_t k = next + lanes_below(idle);
-            next += (uint32_t)__popcll(idle);
-            if (ray == kNone && k < end) {
-                float3 o, d;
-                load(k, o, d, tmx);
-                if (trav_init(T, sc, o, d)) ray = k;
-                else store(k, T, false);
-            }
-        }
-        if (__ballot(ray != kNone) == 0) {
-            if (!more && next >= end) break;
-            continue;
-        }
-        if (ray != kNone) {
-            if (trav_step<kStack, EXH, false, ANY, false>(T, sc, tmx, stk, kRBlock, cnt)) {
-                store(ray, T, true);
-                ray = kNone;
-            }
-        }
-    }
+// Caller rays (tn_trace): ray i accumulates into accum[i]; seeds from (seed, i).
+__global__ __launch_bounds__(kRBlock) void k_raygen_rays(RenderArgs A, Queues Q, const float* o3,
+                                                         const float* d3, uint64_t first,
+                                                         uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint64_t j = first + i;
+    Q.o[0][i] = make_float4(o3[3 * j], o3[3 * j + 1], o3[3 * j + 2], __uint_as_float((uint32_t)j));
+    Q.d[0][i] = make_float4(d3[3 * j], d3[3 * j + 1], d3[3 * j + 2], __uint_as_float(0u));
+    Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
+    Q.rng[0][i] = rng_seed(A.seed, j);
 }
 
+// Per-wave dynamic chunks of a device-counted queue (one atomic per chunk).
+struct DynamicChunks {
+    uint32_t n;
+    uint32_t* work;
+    __device__ bool operator()(uint32_t& b, uint32_t& e) const {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(work, (uint32_t)kChunk);
+        base = __shfl(base, 0, 64);
+        if (base >= n) return false;
+        b = base;
+        e = min(base + (uint32_t)kChunk, n);
+        return true;
+    }
+};
+
 template <bool EXH>
-__global__ __launch_bounds__(kRBlock) void k_trace(RenderArgs A, Queues Q, int level, int buf) {
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
+    RenderArgs A, Queues Q, int level, int buf) {
     __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[level], Q.cap);  // overflow is reported, never read
     const float4* qo = Q.o[buf];
     const float4* qd = Q.d[buf];
     uint4* hit = Q.hit;
-    refill_traverse<EXH, false>(
-        A.sv, n, Q.cnt + 2 * kMaxLevels + level, stk + threadIdx.x,
+    TraversalCounters cnt;
+    coop_loop<kStack, EXH, false, false, false>(
+        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        DynamicChunks{n, Q.cnt + 2 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float&) {
             const float4 a = qo[k], b = qd[k];
             o = make_float3(a.x, a.y, a.z);
             d = make_float3(b.x, b.y, b.z);
         },
-        [&](uint32_t k, const Trav& T, bool) {
-            const bool h = T.best_id != kInvalidId;
-            hit[k] = make_uint4(T.best_id, __float_as_uint(h ? T.best_r : FLT_MAX),
+        [&](uint32_t k, const Trav& T, bool traversed) {
+            const bool h = traversed && T.best_id != kInvalidId;
+            hit[k] = make_uint4(h ? T.best_id : kInvalidId, __float_as_uint(h ? T.best_r : FLT_MAX),
                                 __float_as_uint(h ? T.best_s : 0.f), __float_as_uint(h ? T.best_t : 0.f));
-        });
+        },
+        cnt);
 }
 
 template <bool EXH>
-__global__ __launch_bounds__(kRBlock) void k_shadow(RenderArgs A, Queues Q, int level) {
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
+    RenderArgs A, Queues Q, int level) {
     __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
-    refill_traverse<EXH, true>(
-        A.sv, n, Q.cnt + 3 * kMaxLevels + level, stk + threadIdx.x,
+    TraversalCounters cnt;
+    coop_loop<kStack, EXH, false, true, false>(
+        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float& tmax) {
             const float4 a = Q.so[k], b = Q.sd[k];
             o = make_float3(a.x, a.y, a.z);
@@ -263,7 +253,8 @@ __global__ __launch_bounds__(kRBlock) void k_shadow(RenderArgs A, Queues Q, int 
         [&](uint32_t k, const Trav& T, bool traversed) {
             if (!traversed || T.best_id == kInvalidId)
                 accumulate(A.accum, __float_as_uint(Q.so[k].w), Q.sc[k]);
-        });
+        },
+        cnt);
 }
 
 // Orthonormal basis around n (Duff et al. 2017, "Building an Orthonormal
@@ -515,14 +506,17 @@ int alloc_queues(RenderBuffers& rb, uint32_t cap) {
 // Renders params->width x height x spp samples, adding the per-pixel SUM of
 // sample radiance into accum (device, RGBA).
 int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
-                const tn_render_params* P, float* accum, tn_render_stats* st, hipStream_t stream) {
-    if (P->width == 0 || P->height == 0 || P->pixel_samples < 1 || P->max_depth < 0 ||
+                const tn_render_params* P, float* accum, tn_render_stats* st, hipStream_t stream,
+                const float* ray_o = nullptr, const float* ray_d = nullptr, uint64_t nrays = 0) {
+    const bool rays = ray_o != nullptr;
+    if ((!rays && (P->width == 0 || P->height == 0 || P->pixel_samples < 1)) || P->max_depth < 0 ||
         P->mc_samples < 0)
         return set_error(TN_ERR_INVALID, "invalid render parameters");
     if (nl > 1) return set_error(TN_ERR_LIMIT, "at most one point light (main.cpp:123)");
     if ((uint32_t)P->max_depth + 2 > kMaxLevels) return set_error(TN_ERR_LIMIT, "max_depth too large");
-    if ((size_t)P->width * P->height >= (1ull << 32))
+    if (!rays && (size_t)P->width * P->height >= (1ull << 32))
         return set_error(TN_ERR_LIMIT, "image too large");
+    if (rays && nrays >= (1ull << 32)) return set_error(TN_ERR_LIMIT, "too many rays");
     const uint32_t m = (uint32_t)P->mc_samples;
     // queue capacity: a chunk's deepest level holds chunk * m^max_depth segments
     double fan = 1;
@@ -548,13 +542,15 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     A.emit_cdf = sc->d_emit_cdf;
     A.nemit = sc->nemit;
     A.emit_area = sc->emit_area;
-    A.cam_pos = make_float4(cam->pos[0], cam->pos[1], cam->pos[2], 0);
-    std::memcpy(A.cam_m, cam->trafo, sizeof A.cam_m);
-    A.cam_dx = cam->delta_x;
-    A.cam_dy = cam->delta_y;
+    if (cam) {
+        A.cam_pos = make_float4(cam->pos[0], cam->pos[1], cam->pos[2], 0);
+        std::memcpy(A.cam_m, cam->trafo, sizeof A.cam_m);
+        A.cam_dx = cam->delta_x;
+        A.cam_dy = cam->delta_y;
+    }
     A.width = P->width;
     A.height = P->height;
-    A.spp = (uint32_t)P->pixel_samples;
+    A.spp = rays ? 1u : (uint32_t)P->pixel_samples;
     A.max_depth = P->max_depth;
     A.mc = P->mc_samples;
     A.flags = P->flags;
@@ -568,7 +564,7 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     }
     A.accum = (float4*)accum;
     float* d_jit = nullptr;
-    if (P->flags & TN_RENDER_REF_JITTER) {
+    if (!rays && (P->flags & TN_RENDER_REF_JITTER)) {
         std::vector<float> jit = reference_jitter(P->width, A.spp);
         TN_HIP(hipMalloc(&d_jit, jit.size() * sizeof(float)));
         TN_HIP(hipMemcpyAsync(d_jit, jit.data(), jit.size() * sizeof(float), hipMemcpyHostToDevice, stream));
@@ -576,7 +572,7 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     }
     k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals);
     const bool exh = P->flags & TN_RENDER_EXHAUSTIVE;
-    const uint64_t total = (uint64_t)P->width * P->height * A.spp;
+    const uint64_t total = rays ? nrays : (uint64_t)P->width * P->height * A.spp;
     const int levels = P->max_depth + 1;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sc->device);
@@ -585,7 +581,11 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     for (uint64_t first = 0; first < total; first += chunk) {
         const uint32_t cnt0 = (uint32_t)std::min<uint64_t>(chunk, total - first);
         k_chunk_init<<<1, 4 * kMaxLevels + 64, 0, stream>>>(rb.q.cnt, cnt0);
-        k_raygen<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, first, cnt0);
+        if (rays)
+            k_raygen_rays<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, ray_o, ray_d,
+                                                                                first, cnt0);
+        else
+            k_raygen<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, first, cnt0);
         int buf = 0;
         for (int lv = 0; lv < levels; ++lv) {
             if (exh) k_trace<true><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
@@ -611,7 +611,7 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     TN_HIP(hipMemcpy(&status, sc->d_status, 4, hipMemcpyDeviceToHost));
     if (st) {
         st->rays = tot[0];
-        st->primary_rays = total;
+        st->primary_rays = rays ? 0 : total;
         st->shadow_rays = tot[1];
         st->ms = ms;
     }
@@ -629,6 +629,18 @@ int tn_render_accumulate(tn_scene* sc, const tn_camera* cam, const tn_light* lig
     if (!sc || !cam || !P || !accum || (nl && !lights))
         return set_error(TN_ERR_INVALID, "NULL argument");
     return render_into(sc, cam, lights, nl, P, accum, st, (hipStream_t)stream);
+}
+
+int tn_trace(tn_scene* sc, const float* o3, const float* d3, uint32_t n, int32_t depth,
+             const tn_light* lights, uint32_t nl, const tn_render_params* P, float* rgba,
+             tn_render_stats* st, void* stream) {
+    if (!sc || !P || (n && (!o3 || !d3 || !rgba)) || (nl && !lights) || depth < 0)
+        return set_error(TN_ERR_INVALID, "NULL or invalid argument");
+    if (st) *st = tn_render_stats{};
+    if (n == 0 || depth > P->max_depth) return TN_OK;  // pathtracer.cpp:17-19
+    tn_render_params q = *P;
+    q.max_depth = P->max_depth - depth;  // levels depth..max_depth remain
+    return render_into(sc, nullptr, lights, nl, &q, rgba, st, (hipStream_t)stream, o3, d3, n);
 }
 
 int tn_render(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,

@@ -309,3 +309,22 @@ def reference_furnace(level=3, rho=0.8):
     part = _mesh_part(v, n, (rho, rho, rho))
     return {"verts": part[0], "normals": part[1], "albedo": part[2], "emission": part[3],
             "camera": camera((0.0, 0.0, -4.0), 40.0), "lights": []}
+
+
+def save_tns(scene, path, aspect=1.0):
+    """Writes a scene for the C++ front end (turner_amd/host/pathtracer):
+    "TNSC", version 1, triangle and light counts, verts (9N), normals (9N),
+    albedo (4N), emission (4N) as float32, camera (pos 3, trafo 9, delta_x,
+    delta_y, aspect) and lights (pos 3, rgba 4)."""
+    cam = scene["camera"]
+    lights = scene.get("lights", [])
+    with open(path, "wb") as f:
+        f.write(b"TNSC")
+        f.write(np.array([1, len(scene["verts"]), len(lights)], np.uint32).tobytes())
+        for k, cols in (("verts", 9), ("normals", 9), ("albedo", 4), ("emission", 4)):
+            f.write(np.ascontiguousarray(scene[k], np.float32).reshape(-1, cols).tobytes())
+        f.write(np.concatenate([np.asarray(cam["pos"], np.float32).ravel(),
+                                np.asarray(cam["trafo"], np.float32).ravel(),
+                                np.array([cam["delta_x"], cam["delta_y"], aspect], np.float32)]).tobytes())
+        for p, c in lights:
+            f.write(np.array([*p, *c], np.float32).tobytes())
