@@ -1,0 +1,206 @@
+// tn_guide.h -- learned path-guiding proposal: device-side model description
+// shared by guide.hip (training / inference kernels) and render.hip (guided
+// sampling).  No reference counterpart: the reference samples the hemisphere
+// uniformly (lib/sampling.h:20-32).
+//
+// features(p, n) = [u, sin(2^k pi u_c), cos(2^k pi u_c) for c = x,y,z, k = 0..11, n]
+//                  with u = (p - bmin) / (bmax - bmin)                         (78)
+// MLP 78 -> 64 -> 64 -> 64 -> 40, ReLU hidden layers, BF16 weights and
+// activations as MFMA operands, FP32 accumulation, FP32 master weights.
+// out[5j .. 5j+4] = (logit_j, m_j, kraw_j): 8-lobe von Mises-Fisher mixture
+// with pi = softmax(logit), mu = m / |m|, kappa = softplus(kraw) + 0.01.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tn {
+
+constexpr int kGIn = 78, kGInPad = 80, kGHid = 64, kGOut = 40, kGOutPad = 48, kGLobes = 8;
+constexpr int kGFreq = 12;
+constexpr float kKappaMin = 0.01f, kKappaMax = 1.0e4f;
+// flat FP32 parameter layout (per layer W (out x in, row major) then b)
+constexpr int kGOffW1 = 0, kGOffB1 = 64 * 78;
+constexpr int kGOffW2 = kGOffB1 + 64, kGOffB2 = kGOffW2 + 64 * 64;
+constexpr int kGOffW3 = kGOffB2 + 64, kGOffB3 = kGOffW3 + 64 * 64;
+constexpr int kGOffW4 = kGOffB3 + 64, kGOffB4 = kGOffW4 + 40 * 64;
+constexpr int kGParams = kGOffB4 + 40;  // 15976
+// BF16 copies, padded: W_l as [out_pad][in_pad] (forward A operand) and
+// W_l^T as [in_pad][out_pad] (backward A operand)
+constexpr int kGW1 = 0, kGW2 = kGW1 + 64 * kGInPad, kGW3 = kGW2 + 64 * 64, kGW4 = kGW3 + 64 * 64;
+constexpr int kGWT2 = kGW4 + kGOutPad * 64, kGWT3 = kGWT2 + 64 * 64, kGWT4 = kGWT3 + 64 * 64;
+constexpr int kGBf16 = kGWT4 + 64 * kGOutPad;
+
+struct GuideView {
+    const uint16_t* wbf;  // BF16 weight copies (layout above)
+    const float* theta;   // FP32 master parameters (biases are read from here)
+    float bmin[3], inv_ext[3];
+};
+
+__host__ __device__ __forceinline__ float softplusf(float x) {
+    return x > 20.f ? x : log1pf(expf(x));
+}
+
+// vMF density on the sphere, stable for small and large kappa.
+__device__ __forceinline__ float vmf_pdf(float cosang, float kappa) {
+    return kappa / (6.28318530717958647692f * -expm1f(-2.0f * kappa)) * expf(kappa * (cosang - 1.0f));
+}
+
+struct Mixture {
+    float pi[kGLobes];
+    float mu[kGLobes][3];
+    float kappa[kGLobes];
+};
+
+__device__ __forceinline__ void mixture_from(const float* out, Mixture& M) {
+    float mx = out[0];
+    for (int j = 1; j < kGLobes; ++j) mx = fmaxf(mx, out[5 * j]);
+    float sum = 0.f;
+    for (int j = 0; j < kGLobes; ++j) {
+        M.pi[j] = expf(out[5 * j] - mx);
+        sum += M.pi[j];
+        const float mx3 = out[5 * j + 1], my = out[5 * j + 2], mz = out[5 * j + 3];
+        const float inv = 1.0f / fmaxf(sqrtf(mx3 * mx3 + my * my + mz * mz), 1e-12f);
+        M.mu[j][0] = mx3 * inv;
+        M.mu[j][1] = my * inv;
+        M.mu[j][2] = mz * inv;
+        M.kappa[j] = fminf(softplusf(out[5 * j + 4]) + kKappaMin, kKappaMax);
+    }
+    const float is = 1.0f / sum;
+    for (int j = 0; j < kGLobes; ++j) M.pi[j] *= is;
+}
+
+__device__ __forceinline__ float mixture_pdf(const Mixture& M, float3 w) {
+    float q = 0.f;
+    for (int j = 0; j < kGLobes; ++j)
+        q += M.pi[j] * vmf_pdf(M.mu[j][0] * w.x + M.mu[j][1] * w.y + M.mu[j][2] * w.z, M.kappa[j]);
+    return q;
+}
+
+// Lobe by the CDF of pi (u0), vMF cosine by inverse CDF (u1), azimuth (u2).
+__device__ __forceinline__ float3 mixture_sample(const Mixture& M, float u0, float u1, float u2) {
+    int j = 0;
+    float c = M.pi[0];
+    while (j < kGLobes - 1 && u0 >= c) c += M.pi[++j];
+    const float k = M.kappa[j];
+    const float cw = fminf(1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k);
+    const float sw = sqrtf(fmaxf(0.f, 1.f - cw * cw));
+    float sp, cp;
+    __sincosf(6.28318530717958647692f * u2, &sp, &cp);
+    const float mx = M.mu[j][0], my = M.mu[j][1], mz = M.mu[j][2];
+    const float sign = copysignf(1.0f, mz);
+    const float a = -1.0f / (sign + mz);
+    const float b = mx * my * a;
+    const float3 b1 = make_float3(1.0f + sign * mx * mx * a, sign * b, -sign * mx);
+    const float3 b2 = make_float3(b, sign + my * my * a, -my);
+    return make_float3(cw * mx + sw * (cp * b1.x + sp * b2.x), cw * my + sw * (cp * b1.y + sp * b2.y),
+                       cw * mz + sw * (cp * b1.z + sp * b2.z));
+}
+
+typedef __attribute__((ext_vector_type(4))) float g_f4;
+typedef __attribute__((ext_vector_type(4))) short g_s4;
+
+// FP32 -> BF16, round to nearest even.
+__host__ __device__ __forceinline__ uint16_t to_bf16(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int kGStrideIn = kGInPad + 8;  // LDS row strides (u16), padded against bank conflicts
+constexpr int kGStrideHid = kGHid + 8;
+
+// Features of one sample (lane) into an LDS row of kGInPad BF16 values.
+__device__ __forceinline__ void guide_features(const GuideView& G, float3 p, float3 n, bool active,
+                                               uint16_t* row) {
+    float u[3] = {(p.x - G.bmin[0]) * G.inv_ext[0], (p.y - G.bmin[1]) * G.inv_ext[1],
+                  (p.z - G.bmin[2]) * G.inv_ext[2]};
+    if (!active) u[0] = u[1] = u[2] = 0.f;
+    row[0] = to_bf16(u[0]);
+    row[1] = to_bf16(u[1]);
+    row[2] = to_bf16(u[2]);
+    int k = 3;
+    for (int c = 0; c < 3; ++c) {
+        float f = 1.0f;
+        for (int q = 0; q < kGFreq; ++q) {
+            float sv, cv;
+            sincospif(f * u[c], &sv, &cv);  // sin / cos of 2^q pi u
+            row[k++] = to_bf16(sv);
+            row[k++] = to_bf16(cv);
+            f *= 2.0f;
+        }
+    }
+    row[75] = to_bf16(active ? n.x : 0.f);
+    row[76] = to_bf16(active ? n.y : 0.f);
+    row[77] = to_bf16(active ? n.z : 0.f);
+    row[78] = 0;
+    row[79] = 0;
+}
+
+// One layer H^T = W X^T on the wave's 64 samples (4 tiles of 16) with
+// mfma_f32_16x16x16bf16_1k.  A = W tile (lane: row 16mt + l%16, columns
+// 16kt + 4(l/16) .. +3), B = X^T tile read from the [sample][feature] LDS
+// rows (lane: sample 16nt + l%16, features 16kt + 4(l/16) .. +3), D (lane:
+// outputs 16mt + 4(l/16) + r, sample 16nt + l%16) is exactly the next
+// layer's B operand layout after bias, ReLU and BF16 rounding.
+template <int IN_TILES, int OUT_TILES, bool HIDDEN>
+__device__ __forceinline__ void guide_layer(const uint16_t* __restrict__ W, const float* __restrict__ bias,
+                                            int out_real, const uint16_t* in, int in_stride,
+                                            uint16_t* out_bf, int out_stride, float* out_f32) {
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        g_s4 bfrag[IN_TILES];
+#pragma unroll
+        for (int kt = 0; kt < IN_TILES; ++kt) {
+            const uint16_t* src = in + (16 * nt + row) * in_stride + 16 * kt + 4 * grp;
+            bfrag[kt] = *reinterpret_cast<const g_s4*>(src);
+        }
+#pragma unroll
+        for (int mt = 0; mt < OUT_TILES; ++mt) {
+            g_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < IN_TILES; ++kt) {
+                const g_s4 a = *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfrag[kt], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 16 * mt + 4 * grp + r;
+                const int smp = 16 * nt + row;
+                float v = acc[r] + (o < out_real ? bias[o] : 0.f);
+                if (HIDDEN) {
+                    out_bf[smp * out_stride + o] = to_bf16(fmaxf(v, 0.f));
+                } else if (o < out_real) {
+                    out_f32[smp * kGOut + o] = v;
+                }
+            }
+        }
+    }
+}
+
+// Runs the MLP for the 64 lanes' samples of one wave (block of 64 threads).
+// Writes the 40 outputs of lane l's sample to out[l * 40 ..].  `lds` must
+// hold 64 * (kGStrideIn + kGStrideHid) uint16 values.
+__device__ __forceinline__ void guide_forward_wave(const GuideView& G, float3 p, float3 n, bool active,
+                                                   float* out, uint16_t* lds) {
+    const int l = threadIdx.x & 63;
+    uint16_t* a = lds;                     // [64][kGStrideIn]
+    uint16_t* b = lds + 64 * kGStrideIn;   // [64][kGStrideHid]
+    guide_features(G, p, n, active, a + l * kGStrideIn);
+    __syncthreads();
+    guide_layer<5, 4, true>(G.wbf + kGW1, G.theta + kGOffB1, 64, a, kGStrideIn, b, kGStrideHid, nullptr);
+    __syncthreads();
+    guide_layer<4, 4, true>(G.wbf + kGW2, G.theta + kGOffB2, 64, b, kGStrideHid, a, kGStrideIn, nullptr);
+    __syncthreads();
+    guide_layer<4, 4, true>(G.wbf + kGW3, G.theta + kGOffB3, 64, a, kGStrideIn, b, kGStrideHid, nullptr);
+    __syncthreads();
+    guide_layer<4, 3, false>(G.wbf + kGW4, G.theta + kGOffB4, 40, b, kGStrideHid, nullptr, 0, out);
+    __syncthreads();
+}
+
+}  // namespace tn
