@@ -219,6 +219,73 @@ int tn_tonemap(const float* accum, float* out, uint32_t num_pixels, float inv_sp
  * query the size. */
 long tn_write_ppm(const float* rgba, uint32_t width, uint32_t height, char* buf, size_t cap);
 
+/*
+ * Learned path guiding.  No reference counterpart: the reference samples the
+ * hemisphere uniformly (lib/sampling.h:20-32, pathtracer.cpp:64-101); these
+ * entry points are an addition behind the same render call.
+ *
+ * A tn_guide holds an MLP (78 -> 64 -> 64 -> 64 -> 40, ReLU) mapping a hit
+ * point and its shading normal to an 8-lobe von Mises-Fisher mixture q(w):
+ * FP32 master parameters, Adam state, BF16 operand copies for the MFMA
+ * kernels and the training set the guided renderer fills.  Flat parameter
+ * layout: per layer W (out x in, row major) then b.  Network outputs per
+ * point: 8 x (logit, m.xyz, kraw); pi = softmax(logit), mu = m / |m|,
+ * kappa = min(softplus(kraw) + 0.01, 1e4).
+ */
+typedef struct tn_guide tn_guide;
+#define TN_GUIDE_PARAMS 15976
+#define TN_GUIDE_OUTPUTS 40
+
+/* Positions are encoded relative to the box [bmin, bmax] (the scene box). */
+int tn_guide_create(const float bmin[3], const float bmax[3], uint64_t seed, tn_guide** out);
+int tn_guide_destroy(tn_guide* g);
+/* FP32 master parameters to / from host memory (TN_GUIDE_PARAMS floats);
+ * set_params resets the Adam state. */
+int tn_guide_get_params(tn_guide* g, float* theta);
+int tn_guide_set_params(tn_guide* g, const float* theta);
+/* Adam learning rate (1e-3), mini-batch size (16384, a multiple of 64) and the
+ * one-sample MIS probability of drawing from the guide (0.5). */
+int tn_guide_set_options(tn_guide* g, float lr, uint32_t batch, float mis_guided);
+/* Network outputs (40 per point, device) for n points (device p3, n3). */
+int tn_guide_forward(tn_guide* g, const float* p3, const float* n3, uint32_t n, float* out,
+                     void* stream);
+/* Mixture density q(w | p, n) (device arrays). */
+int tn_guide_pdf(tn_guide* g, const float* p3, const float* n3, const float* w3, uint32_t n,
+                 float* pdf, void* stream);
+/* Directions drawn from q with the uniforms u3 (device, 3 per point: lobe,
+ * cosine, azimuth) and their density q(w). */
+int tn_guide_sample(tn_guide* g, const float* p3, const float* n3, const float* u3, uint32_t n,
+                    float* w3, float* pdf, void* stream);
+/* Loss mean_i(c_i / mean(c) * -log q(w_i | p_i, n_i)) of a batch (device
+ * arrays) and its gradient (device, TN_GUIDE_PARAMS floats).  The loss is
+ * returned to the host (synchronises the stream). */
+int tn_guide_loss_grad(tn_guide* g, const float* p3, const float* n3, const float* w3,
+                       const float* c, uint32_t n, float* grad, float* loss, void* stream);
+/* One Adam step on a batch (device arrays); loss may be NULL (asynchronous). */
+int tn_guide_train_step(tn_guide* g, const float* p3, const float* n3, const float* w3,
+                        const float* c, uint32_t n, float* loss, void* stream);
+/* Samples in the training set of the last training render, Adam steps taken
+ * and the loss of the last step. */
+int tn_guide_info(tn_guide* g, uint32_t* train_samples, uint32_t* steps, float* last_loss);
+/* Copies up to cap samples of that training set to host memory, 12 floats
+ * each: p.xyz, c, n.xyz, cos/pdf, w.xyz, tag (uint32 bits: path index within
+ * its chunk << 6 | bounce level).  Returns the number copied. */
+long tn_guide_get_train_set(tn_guide* g, float* out, size_t cap);
+
+/*
+ * tn_render_accumulate with the guide as proposal: every indirect direction
+ * is drawn from q with probability mis_guided and from the cosine lobe
+ * otherwise, weighted by the exact one-sample MIS density
+ * mis_guided * q(w) + (1 - mis_guided) cos(w) / pi, so the estimator of
+ * trace() stays unbiased.  With train_samples > 0 (needs mc_samples == 1)
+ * the render also records, for every bounce, the radiance its path brought
+ * back, and afterwards runs train_samples / batch Adam steps on mini-batches
+ * drawn from those records.
+ */
+int tn_render_guided(tn_scene* scene, const tn_camera* cam, const tn_light* lights,
+                     uint32_t num_lights, const tn_render_params* params, tn_guide* guide,
+                     uint32_t train_samples, float* accum, tn_render_stats* stats, void* stream);
+
 const char* tn_last_error(void);
 /* Device name and compute-unit count of the current HIP device. */
 int tn_device_info(char* name, size_t cap, int* compute_units);

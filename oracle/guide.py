@@ -35,9 +35,31 @@ def n_params():
     return sum(o * i + o for i, o in LAYERS)
 
 
-def features(p, n, bmin, bmax):
-    p = np.asarray(p, np.float64)
-    u = (p - np.asarray(bmin, np.float64)) / (np.asarray(bmax, np.float64) - np.asarray(bmin, np.float64))
+def bf16(a):
+    """Round to BF16 (nearest even) through FP32, as the device does, and
+    return the values as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+def unit_coords_f32(p, bmin, bmax):
+    """u = (p - bmin) * (1 / (bmax - bmin)) in FP32 operations, as the device
+    computes it before the encoding."""
+    p = np.asarray(p, np.float32)
+    bmin = np.asarray(bmin, np.float32)
+    ext = (np.asarray(bmax, np.float32) - bmin).astype(np.float32)
+    inv = (np.float32(1.0) / ext).astype(np.float32)
+    return ((p - bmin).astype(np.float32) * inv).astype(np.float32)
+
+
+def features(p, n, bmin, bmax, u=None):
+    """Encoding of points p (n, 3) with normals n; u (optional) gives the unit
+    coordinates directly (e.g. from unit_coords_f32)."""
+    if u is None:
+        p = np.asarray(p, np.float64)
+        u = (p - np.asarray(bmin, np.float64)) / (np.asarray(bmax, np.float64) - np.asarray(bmin, np.float64))
+    u = np.asarray(u, np.float64)
     cols = [u]
     for c in range(3):
         for k in range(N_FREQ):
@@ -60,14 +82,21 @@ def unpack(theta):
     return out
 
 
-def forward(theta, x):
-    acts = [x]
-    h = x
+def forward(theta, x, quant=False):
+    """MLP forward.  quant=True rounds every product operand (layer inputs and
+    weights) to BF16 as the MFMA kernels do; sums stay exact (FP64).
+    Returns (outputs, [layer inputs..., outputs]) -- the layer inputs as the
+    products saw them."""
+    q = bf16 if quant else (lambda a: a)
     layers = unpack(theta)
+    acts = []
+    h = x
     for li, (W, b) in enumerate(layers):
-        z = h @ W.T + b
+        hin = q(h)
+        acts.append(hin)
+        z = hin @ q(W).T + b
         h = np.maximum(z, 0.0) if li < len(layers) - 1 else z
-        acts.append(h)
+    acts.append(h)
     return h, acts
 
 
@@ -96,17 +125,23 @@ def pdf(out, w):
     return np.sum(pi * vmf_pdf(w, mu, kappa), axis=1)
 
 
-def loss_and_grad(theta, x, w, c):
-    """Returns (loss, dloss/dtheta, dloss/dout)."""
-    out, acts = forward(theta, x)
-    B = len(x)
+def loss_and_grad(theta, x, w, c, quant=False):
+    """Returns (loss, dloss/dtheta, dloss/dout).  The per-sample gradients
+    carry the weight c_i and the sums are divided by sum(c) at the end
+    (mean_i(c_i / mean(c) * l_i) = sum_i c_i l_i / sum(c)).  quant=True
+    rounds every product operand to BF16 as the MFMA kernels do: layer
+    inputs, weights and the back-propagated deltas (whose BF16 values also
+    form the bias gradients)."""
+    q = bf16 if quant else (lambda a: a)
+    out, acts = forward(theta, x, quant)
     pi, mu, kappa, mnorm = mixture(out)
     v = vmf_pdf(w, mu, kappa)
-    q = np.sum(pi * v, axis=1)
-    cw = c / np.mean(c)
-    loss = np.mean(cw * -np.log(q))
-    g = (cw / B)[:, None]  # d loss / d(-log q)
-    gamma = pi * v / q[:, None]
+    qv = np.sum(pi * v, axis=1)
+    csum = np.sum(c)
+    loss = np.sum(c * -np.log(qv)) / csum
+    q_ = qv
+    g = c[:, None]  # d (c * -log q) / d(-log q)
+    gamma = pi * v / q_[:, None]
     cos = np.einsum("bc,blc->bl", w, mu)
     d_logit = -(gamma - pi) * g
     d_mu = -(gamma * kappa)[..., None] * w[:, None, :] * g[..., None]
@@ -119,20 +154,20 @@ def loss_and_grad(theta, x, w, c):
     d_kappa = -gamma * dlogv_dk * g
     sig = 1.0 / (1.0 + np.exp(-kraw))
     d_kraw = d_kappa * sig * (kk < KAPPA_MAX)
-    d_out = np.concatenate([d_logit[..., None], d_m, d_kraw[..., None]], axis=2).reshape(B, N_OUT)
+    d_out = np.concatenate([d_logit[..., None], d_m, d_kraw[..., None]], axis=2).reshape(len(x), N_OUT)
     # backprop through the MLP
     layers = unpack(theta)
     grads = []
     delta = d_out
     for li in range(len(layers) - 1, -1, -1):
         W, b = layers[li]
-        a_in = acts[li]
-        grads.append((delta.T @ a_in, delta.sum(0)))
+        dq = q(delta)
+        grads.append((dq.T @ acts[li], dq.sum(0)))
         if li > 0:
-            delta = (delta @ W) * (acts[li] > 0)
+            delta = (dq @ q(W)) * (acts[li] > 0)
     grads = grads[::-1]
     flat = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads])
-    return loss, flat, d_out
+    return loss, flat / csum, d_out / csum
 
 
 def adam(theta, grad, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

@@ -93,3 +93,21 @@ def test_adam_reduces_the_loss():
         loss, g, _ = G.loss_and_grad(theta, x, w, c)
         theta, m, v = G.adam(theta, g, m, v, t, lr=3e-3)
     assert loss < l0 - 1.0
+
+
+def test_bf16_rounding():
+    """Round to nearest even at 8 significant bits, through FP32."""
+    one = 1.0
+    cases = {one: one, one + 2 ** -8: one, one + 3 * 2 ** -8: one + 2 ** -6,
+             one + 2 ** -8 + 2 ** -20: one + 2 ** -7, -2.5: -2.5, 0.0: 0.0,
+             3.0e38: float(np.array([0x7F620000], np.uint32).view(np.float32)[0])}  # 0x7F61B1E6 rounds up
+    for x, want in cases.items():
+        assert G.bf16(np.array([x]))[0] == np.float64(np.float32(want)), x
+    # the quantised model is the FP64 model on BF16-representable inputs
+    th = G.bf16(G.init_params(1))
+    x = G.bf16(np.random.default_rng(0).normal(size=(16, G.N_IN)))
+    a, _ = G.forward(th, x)
+    b, _ = G.forward(th, x, quant=True)
+    np.testing.assert_allclose(a[:, :], b[:, :], rtol=0, atol=0.02)
+    h0, acts = G.forward(th, x, quant=True)
+    assert np.array_equal(acts[0], x)

@@ -23,7 +23,7 @@ from ._lib import TN_INVALID_ID, TurnerError, check, lib, ptr
 
 __all__ = ["KDTree", "KDTreeIntersection", "TracerConfig", "render", "TurnerError",
            "TN_INVALID_ID", "device_info", "build_tree", "tree_cache_save", "tree_cache_load",
-           "write_ppm"]
+           "write_ppm", "render_accumulate"]
 
 
 def build_tree(verts, serial=False):
@@ -264,6 +264,26 @@ def render(tree, cam, lights, conf, display=False):
     stats = {"rays": st.rays, "primary_rays": st.primary_rays, "shadow_rays": st.shadow_rays,
              "ms": st.ms}
     return (lin, stats, disp) if display else (lin, stats)
+
+
+def render_accumulate(tree, cam, lights, conf, accum, sample_offset=0):
+    """tn_render_accumulate: adds the per-pixel radiance SUMS of
+    conf.num_pixel_samples samples (pixel-sample indices from sample_offset,
+    fresh random streams per offset) into accum, a (h, w, 4) float32 device
+    tensor.  Asynchronous on torch's current stream; returns the stats."""
+    import torch
+    w, h = conf.width, conf.height()
+    if not (isinstance(accum, torch.Tensor) and accum.is_cuda and accum.dtype == torch.float32
+            and accum.is_contiguous() and accum.numel() == w * h * 4):
+        raise ValueError("accum must be a contiguous float32 device tensor of h * w * 4")
+    p = _params(conf, w, h)
+    p.sample_offset = sample_offset
+    st = _lib.RenderStats()
+    stream = torch.cuda.current_stream(accum.device).cuda_stream
+    check(lib().tn_render_accumulate(tree.handle, C.byref(_camera(cam)), _lights(lights), len(lights),
+                                     C.byref(p), ptr(accum), C.byref(st), stream))
+    return {"rays": st.rays, "primary_rays": st.primary_rays, "shadow_rays": st.shadow_rays,
+            "ms": st.ms}
 
 
 def write_ppm(rgba):

@@ -18,6 +18,8 @@ TN_ERR_NODEVICE = -5
 TN_ERR_LIMIT = -6
 TN_ERR_INCOMPLETE = -7
 TN_INVALID_ID = 1 << 30
+TN_GUIDE_PARAMS = 15976
+TN_GUIDE_OUTPUTS = 40
 
 TN_SCENE_SERIAL_BUILD = 1
 TN_ISECT_EXHAUSTIVE = 1
@@ -99,6 +101,21 @@ SIGNATURES = {
     "tn_tree_cache_save": (C.c_int, [C.c_char_p, _P, _U32, _P, C.c_size_t, _P]),
     "tn_tree_cache_load": (C.c_int, [C.c_char_p, _P, _U32, C.POINTER(C.POINTER(C.c_uint64)),
                                      C.POINTER(C.c_size_t), _P]),
+    "tn_guide_create": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(_P)]),
+    "tn_guide_destroy": (C.c_int, [_P]),
+    "tn_guide_get_params": (C.c_int, [_P, _P]),
+    "tn_guide_set_params": (C.c_int, [_P, _P]),
+    "tn_guide_set_options": (C.c_int, [_P, C.c_float, _U32, C.c_float]),
+    "tn_guide_forward": (C.c_int, [_P, _P, _P, _U32, _P, _P]),
+    "tn_guide_pdf": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P]),
+    "tn_guide_sample": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _P]),
+    "tn_guide_loss_grad": (C.c_int, [_P, _P, _P, _P, _P, _U32, _P, _FP, _P]),
+    "tn_guide_train_step": (C.c_int, [_P, _P, _P, _P, _P, _U32, _FP, _P]),
+    "tn_guide_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), _FP]),
+    "tn_guide_get_train_set": (C.c_long, [_P, _P, C.c_size_t]),
+    "tn_render_guided": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(Light), _U32,
+                                   C.POINTER(RenderParams), _P, _U32, _P,
+                                   C.POINTER(RenderStats), _P]),
     "tn_last_error": (C.c_char_p, []),
     "tn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
 }

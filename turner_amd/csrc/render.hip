@@ -27,6 +27,18 @@
 // with the reference estimator statistically.  With TN_RENDER_REF_JITTER the
 // pixel jitter is the reference's exactly (xorshift64* seeded 42 per row,
 // main.cpp:201-206), so primary rays are bit-identical.
+//
+// Guided rendering (tn_render_guided, no reference counterpart) swaps the
+// hemisphere sampler in shade for one-sample MIS between the learned vMF
+// mixture q and the cosine lobe: the wave runs the guide MLP on its 64 hits
+// (BF16 MFMA, tn_guide.h), each child direction comes from q with
+// probability a and from the cosine lobe otherwise, and carries the weight
+// rho cos / (pi m (a q(w) + (1 - a) cos / pi)), so E[estimate] is unchanged.
+// When training, every bounce of a path (m = 1) leaves a record (point,
+// normal, direction, cos / pdf, rho f) and every segment the radiance it
+// found itself (background, unoccluded next-event estimates); after the
+// chunk, k_collect walks each path backwards, L_k = X_{k+1} + g_{k+1} L_{k+1},
+// and appends (p, n, w, lum(L_k) cos / pdf) to the guide's training set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +48,7 @@
 #include <mutex>
 #include <vector>
 
+#include "tn_guide.h"
 #include "tn_scene.h"
 
 namespace tn {
@@ -53,6 +66,7 @@ struct Queues {
     float4* so;     // shadow: origin, pixel (bits)
     float4* sd;     // shadow: direction, tmax
     float4* sc;     // shadow: contribution
+    float4* sx;     // shadow: radiance at the segment (training records), slot (bits)
     uint32_t cap;   // capacity of each path queue (entries)
     uint32_t scap;  // capacity of the shadow queue
     // device counters: [0..kMaxLevels) path counts, [kMaxLevels..2k) shadow
@@ -83,6 +97,21 @@ struct RenderArgs {
     float4 light_pos, light_col;
     const float* jitter;  // 2 * width * spp (REF_JITTER) or null
     float4* accum;
+    uint64_t seed_pass;  // seed of this call's sample range (sample_offset)
+    // guided sampling
+    GuideView gv;
+    float mis;  // probability of drawing from the guide
+    // training records, slot = path * rstride + level (null unless training)
+    float4* rec_p;  // hit point, cos / pdf
+    float4* rec_n;  // shading normal, valid (1) when the bounce spawned its child
+    float4* rec_w;  // direction
+    float4* rec_g;  // rho * f: the child's weight relative to its parent
+    float4* rec_x;  // radiance found by the segment itself (its level), local units
+    uint32_t rstride;
+    float4* ts;  // guide training set (3 float4 per sample)
+    uint32_t* ts_count;
+    uint32_t ts_cap;
+    float accept;  // probability that a record enters the training set
 };
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
@@ -155,7 +184,7 @@ __global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q, uint
     const uint32_t sample = (uint32_t)(j % A.spp);
     const uint64_t pix = j / A.spp;
     const uint32_t x = (uint32_t)(pix % A.width), y = (uint32_t)(pix / A.width);
-    uint4 rng = rng_seed(A.seed, j);
+    uint4 rng = rng_seed(A.seed_pass, j);
     float dx, dy;
     if (A.jitter) {
         const size_t k = 2 * ((size_t)x * A.spp + sample);
@@ -175,7 +204,7 @@ __global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q, uint
     const float3 d = make_float3(m[0] * cx + m[1] * cy + m[2] * cz, m[3] * cx + m[4] * cy + m[5] * cz,
                                  m[6] * cx + m[7] * cy + m[8] * cz);
     Q.o[0][i] = make_float4(A.cam_pos.x, A.cam_pos.y, A.cam_pos.z, __uint_as_float((uint32_t)pix));
-    Q.d[0][i] = make_float4(d.x, d.y, d.z, __uint_as_float(0u));
+    Q.d[0][i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));  // .w: path within the chunk
     Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
     Q.rng[0][i] = rng;
 }
@@ -188,9 +217,9 @@ __global__ __launch_bounds__(kRBlock) void k_raygen_rays(RenderArgs A, Queues Q,
     if (i >= count) return;
     const uint64_t j = first + i;
     Q.o[0][i] = make_float4(o3[3 * j], o3[3 * j + 1], o3[3 * j + 2], __uint_as_float((uint32_t)j));
-    Q.d[0][i] = make_float4(d3[3 * j], d3[3 * j + 1], d3[3 * j + 2], __uint_as_float(0u));
+    Q.d[0][i] = make_float4(d3[3 * j], d3[3 * j + 1], d3[3 * j + 2], __uint_as_float(i));
     Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
-    Q.rng[0][i] = rng_seed(A.seed, j);
+    Q.rng[0][i] = rng_seed(A.seed_pass, j);
 }
 
 // Per-wave dynamic chunks of a device-counted queue (one atomic per chunk).
@@ -251,8 +280,16 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
             tmax = b.w;
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
-            if (!traversed || T.best_id == kInvalidId)
+            if (!traversed || T.best_id == kInvalidId) {
                 accumulate(A.accum, __float_as_uint(Q.so[k].w), Q.sc[k]);
+                if (A.rec_x) {
+                    const float4 x = Q.sx[k];
+                    float* dst = reinterpret_cast<float*>(A.rec_x + __float_as_uint(x.w));
+                    atomicAdd(dst + 0, x.x);
+                    atomicAdd(dst + 1, x.y);
+                    atomicAdd(dst + 2, x.z);
+                }
+            }
         },
         cnt);
 }
@@ -267,7 +304,20 @@ __device__ __forceinline__ void onb(float3 n, float3& b1, float3& b2) {
     b2 = make_float3(b, sign + n.y * n.y * a, -n.y);
 }
 
+__device__ __forceinline__ void push_shadow(const Queues& Q, uint32_t k, float3 o, uint32_t pixel, float3 dir,
+                                            float tmax, float4 contrib, float4 local, uint32_t slot,
+                                            bool records) {
+    Q.so[k] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
+    Q.sd[k] = make_float4(dir.x, dir.y, dir.z, tmax);
+    Q.sc[k] = contrib;
+    if (records) Q.sx[k] = make_float4(local.x, local.y, local.z, __uint_as_float(slot));
+}
+
+// Shade one level.  GUIDED kernels run one wave per block (the guide's
+// forward pass is separated by block barriers).
+template <bool GUIDED>
 __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ __align__(16) uint16_t glds[GUIDED ? 64 * (kGStrideIn + kGStrideHid) : 1];
     const uint32_t n = min(Q.cnt[level], Q.cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -276,30 +326,34 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
     const uint32_t m = (uint32_t)max(A.mc, 0);
     const bool cosine = A.flags & TN_RENDER_COSINE;
     const bool area = (A.flags & TN_RENDER_AREA_LIGHTS) && A.nemit > 0;
+    const bool records = GUIDED && A.rec_p != nullptr;
+    const float inv_pi = 0.318309886183790671538f;
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane;
         const bool active = i < n;
         bool has_hit = false, has_point = false, has_area = false;
         float4 w = make_float4(0, 0, 0, 0), rho = w;
-        float3 p2 = make_float3(0, 0, 0), nn = p2, ldir = p2, adir = p2;
+        float3 p = make_float3(0, 0, 0), p2 = p, nn = p, ldir = p, adir = p;
         float ldist = 0.f, adist = 0.f;
-        float4 lcontrib = w, acontrib = w;
-        uint32_t pixel = 0;
+        float4 lcontrib = w, acontrib = w, llocal = w, alocal = w;
+        uint32_t pixel = 0, slot = 0;
         uint4 rng = make_uint4(1, 0, 0, 0);
         if (active) {
             const float4 qo = Q.o[buf][i], qd = Q.d[buf][i];
             w = Q.w[buf][i];
             rng = Q.rng[buf][i];
             pixel = __float_as_uint(qo.w);
+            slot = __float_as_uint(qd.w) * A.rstride + (uint32_t)level;
             const uint4 h = Q.hit[i];
             if (h.x == kInvalidId) {
                 accumulate(A.accum, pixel, mul4(w, A.bg));  // pathtracer.cpp:26-28
+                if (records) A.rec_x[slot] = A.bg;
             } else {
                 has_hit = true;
                 const uint32_t id = h.x;
                 const float r = __uint_as_float(h.y), s = __uint_as_float(h.z), t = __uint_as_float(h.w);
                 // pathtracer.cpp:30-36
-                const float3 p = make_float3(qo.x + r * qd.x, qo.y + r * qd.y, qo.z + r * qd.z);
+                p = make_float3(qo.x + r * qd.x, qo.y + r * qd.y, qo.z + r * qd.z);
                 const float* nv = A.normals + 9 * (size_t)id;
                 const float b0 = 1.f - s - t;
                 float3 ni = make_float3(b0 * nv[0] + s * nv[3] + t * nv[6], b0 * nv[1] + s * nv[4] + t * nv[7],
@@ -309,7 +363,6 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                 p2 = make_float3(p.x + 0.0001f * nn.x, p.y + 0.0001f * nn.y, p.z + 0.0001f * nn.z);
                 rho = A.albedo[id];
                 const float4 wr = mul4(w, rho);
-                const float inv_pi = 0.318309886183790671538f;
                 if (A.nlights) {  // pathtracer.cpp:44-58
                     float3 l = make_float3(A.light_pos.x - p.x, A.light_pos.y - p.y, A.light_pos.z - p.z);
                     const float il = 1.0f / sqrtf(l.x * l.x + l.y * l.y + l.z * l.z);
@@ -318,6 +371,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                     ldist = sqrtf(l2.x * l2.x + l2.y * l2.y + l2.z * l2.z);
                     const float c = fmaxf(0.f, ldir.x * nn.x + ldir.y * nn.y + ldir.z * nn.z);
                     lcontrib = scale4(inv_pi, mul4(wr, scale4(c, A.light_col)));
+                    if (records) llocal = scale4(inv_pi, mul4(rho, scale4(c, A.light_col)));
                     has_point = true;
                 }
                 if (area) {
@@ -354,6 +408,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                         const float4 lek = A.emission[k];
                         const float gterm = cosx * cosl / d2 * A.emit_area;
                         acontrib = scale4(inv_pi, mul4(wr, scale4(gterm, lek)));
+                        if (records) alocal = scale4(inv_pi, mul4(rho, scale4(gterm, lek)));
                         has_area = true;
                     }
                 }
@@ -368,73 +423,176 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
             sbase = __shfl(sbase, 0);
             if (has_point) {
                 const uint32_t k = sbase + lanes_below(mp);
-                if (k < Q.scap) {
-                    Q.so[k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
-                    Q.sd[k] = make_float4(ldir.x, ldir.y, ldir.z, ldist);
-                    Q.sc[k] = lcontrib;
+                if (k < Q.scap) push_shadow(Q, k, p2, pixel, ldir, ldist, lcontrib, llocal, slot, records);
+                else atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
+            }
+            if (has_area) {
+                const uint32_t k = sbase + (uint32_t)__popcll(mp) + lanes_below(ma);
+                if (k < Q.scap) push_shadow(Q, k, p2, pixel, adir, adist, acontrib, alocal, slot, records);
+                else atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
+            }
+        }
+        if (!spawn) continue;
+        const int ob = buf ^ 1;
+        const float invm = 1.0f / (float)m;
+        const float4 wr = mul4(w, rho);
+        const float pathbits = Q.d[buf][active ? i : 0].w;
+        float3 b1, b2;
+        onb(nn, b1, b2);
+        if (!GUIDED) {
+            // children: m per hit (pathtracer.cpp:72-85)
+            const uint64_t mc = __ballot(has_hit);
+            if (!mc) continue;
+            uint32_t cbase = 0;
+            if (lane == 0) cbase = atomicAdd(&Q.cnt[level + 1], (uint32_t)__popcll(mc) * m);
+            cbase = __shfl(cbase, 0);
+            if (!has_hit) continue;
+            const uint32_t first = cbase + lanes_below(mc) * m;
+            for (uint32_t c = 0; c < m; ++c) {
+                const float u1 = rng_uniform(rng), u2 = rng_uniform(rng);
+                float lx, ly, lz, f;
+                const float phi = 6.28318530717958647692f * u2;
+                float sp, cp;
+                __sincosf(phi, &sp, &cp);
+                if (cosine) {
+                    const float rr = sqrtf(u1);
+                    lx = rr * cp;
+                    ly = rr * sp;
+                    lz = sqrtf(fmaxf(0.f, 1.f - u1));
+                    f = invm;
+                } else {  // lib/sampling.h:20-32
+                    lz = u1;
+                    const float rr = sqrtf(fmaxf(0.f, 1.f - lz * lz));
+                    lx = rr * cp;
+                    ly = rr * sp;
+                    f = 2.f * u1 * invm;
+                }
+                const float3 dir = make_float3(lx * b1.x + ly * b2.x + lz * nn.x, lx * b1.y + ly * b2.y + lz * nn.y,
+                                               lx * b1.z + ly * b2.z + lz * nn.z);
+                const uint32_t k = first + c;
+                if (k < Q.cap) {
+                    Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
+                    Q.d[ob][k] = make_float4(dir.x, dir.y, dir.z, pathbits);
+                    Q.w[ob][k] = scale4(f, wr);
+                    Q.rng[ob][k] = rng_seed(((uint64_t)rng_next(rng) << 32) | rng_next(rng), c);
                 } else {
                     atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
                 }
             }
-            if (has_area) {
-                const uint32_t k = sbase + (uint32_t)__popcll(mp) + lanes_below(ma);
-                if (k < Q.scap) {
-                    Q.so[k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
-                    Q.sd[k] = make_float4(adir.x, adir.y, adir.z, adist);
-                    Q.sc[k] = acontrib;
+        } else {
+            // guided: the wave's 64 hits through the guide MLP
+            float* gout = reinterpret_cast<float*>(glds);
+            guide_forward_wave(A.gv, p, nn, has_hit, gout, glds);
+            Mixture M;
+            mixture_from(gout + lane * kGOut, M);
+            __syncthreads();  // gout is overwritten by the next iteration's forward pass
+            for (uint32_t c = 0; c < m; ++c) {
+                const float us = rng_uniform(rng), u0 = rng_uniform(rng), u1 = rng_uniform(rng),
+                            u2 = rng_uniform(rng);
+                float3 dir;
+                if (us < A.mis) {
+                    dir = mixture_sample(M, u0, u1, u2);
+                } else {
+                    const float rr = sqrtf(u1);
+                    float sp, cp;
+                    __sincosf(6.28318530717958647692f * u2, &sp, &cp);
+                    const float lx = rr * cp, ly = rr * sp, lz = sqrtf(fmaxf(0.f, 1.f - u1));
+                    dir = make_float3(lx * b1.x + ly * b2.x + lz * nn.x, lx * b1.y + ly * b2.y + lz * nn.y,
+                                      lx * b1.z + ly * b2.z + lz * nn.z);
+                }
+                const float cosn = dir.x * nn.x + dir.y * nn.y + dir.z * nn.z;
+                const bool ok = has_hit && cosn > 0.f;
+                float f = 0.f, cos_pdf = 0.f;
+                if (ok) {
+                    const float pdf = A.mis * mixture_pdf(M, dir) + (1.0f - A.mis) * cosn * inv_pi;
+                    cos_pdf = cosn / pdf;
+                    f = inv_pi * cos_pdf * invm;
+                }
+                const uint64_t mc = __ballot(ok);
+                uint32_t cbase = 0;
+                if (mc && lane == 0) cbase = atomicAdd(&Q.cnt[level + 1], (uint32_t)__popcll(mc));
+                cbase = __shfl(cbase, 0);
+                if (records && has_hit) {
+                    A.rec_p[slot] = make_float4(p.x, p.y, p.z, cos_pdf);
+                    A.rec_n[slot] = make_float4(nn.x, nn.y, nn.z, ok ? 1.f : 0.f);
+                    A.rec_w[slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
+                    A.rec_g[slot] = scale4(f, rho);
+                }
+                if (!ok) continue;
+                const uint32_t k = cbase + lanes_below(mc);
+                if (k < Q.cap) {
+                    Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
+                    Q.d[ob][k] = make_float4(dir.x, dir.y, dir.z, pathbits);
+                    Q.w[ob][k] = scale4(f, wr);
+                    Q.rng[ob][k] = rng_seed(((uint64_t)rng_next(rng) << 32) | rng_next(rng), c);
                 } else {
                     atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
                 }
             }
         }
-        // children: m per hit (pathtracer.cpp:72-85)
-        if (spawn) {
-            const uint64_t mc = __ballot(has_hit);
-            if (mc) {
-                uint32_t cbase = 0;
-                if (lane == 0) cbase = atomicAdd(&Q.cnt[level + 1], (uint32_t)__popcll(mc) * m);
-                cbase = __shfl(cbase, 0);
-                if (has_hit) {
-                    float3 b1, b2;
-                    onb(nn, b1, b2);
-                    const float4 wr = mul4(w, rho);
-                    const uint32_t first = cbase + lanes_below(mc) * m;
-                    const int ob = buf ^ 1;
-                    const float invm = 1.0f / (float)m;
-                    for (uint32_t c = 0; c < m; ++c) {
-                        const float u1 = rng_uniform(rng), u2 = rng_uniform(rng);
-                        float lx, ly, lz, f;
-                        const float phi = 6.28318530717958647692f * u2;
-                        float sp, cp;
-                        __sincosf(phi, &sp, &cp);
-                        if (cosine) {
-                            const float rr = sqrtf(u1);
-                            lx = rr * cp;
-                            ly = rr * sp;
-                            lz = sqrtf(fmaxf(0.f, 1.f - u1));
-                            f = invm;
-                        } else {  // lib/sampling.h:20-32
-                            lz = u1;
-                            const float rr = sqrtf(fmaxf(0.f, 1.f - lz * lz));
-                            lx = rr * cp;
-                            ly = rr * sp;
-                            f = 2.f * u1 * invm;
-                        }
-                        const float3 dir = make_float3(lx * b1.x + ly * b2.x + lz * nn.x,
-                                                       lx * b1.y + ly * b2.y + lz * nn.y,
-                                                       lx * b1.z + ly * b2.z + lz * nn.z);
-                        const uint32_t k = first + c;
-                        if (k < Q.cap) {
-                            Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
-                            Q.d[ob][k] = make_float4(dir.x, dir.y, dir.z, __uint_as_float((uint32_t)level + 1));
-                            Q.w[ob][k] = scale4(f, wr);
-                            Q.rng[ob][k] = rng_seed(((uint64_t)rng_next(rng) << 32) | rng_next(rng), c);
-                        } else {
-                            atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
-                        }
-                    }
+    }
+}
+
+// Clears the training records of a chunk: valid flags and found radiance.
+__global__ void k_rec_clear(RenderArgs A, uint32_t slots) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= slots) return;
+    A.rec_n[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A.rec_x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Walks each path of the chunk backwards (L_k = X_{k+1} + g_{k+1} L_{k+1})
+// and appends (p, n, w, lum(L_k) cos / pdf) of every bounce that brought
+// light back to the training set (kept with probability A.accept).  Sample
+// layout: (p, c), (n, cos / pdf), (w, path << 6 | level as bits).
+__global__ __launch_bounds__(kRBlock) void k_collect(RenderArgs A, uint32_t npaths, int levels_with_records,
+                                                     uint64_t salt) {
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool act = pid < npaths;
+    float3 L = make_float3(0.f, 0.f, 0.f), gprev = L;
+    bool have = false;
+    for (int k = levels_with_records - 1; k >= 0; --k) {
+        const uint32_t slot = (act ? pid : 0u) * A.rstride + (uint32_t)k;
+        bool emit = false;
+        float4 rp, rn, rw;
+        float c = 0.f;
+        if (act) {
+            rn = A.rec_n[slot];
+            if (rn.w == 0.f) {
+                have = false;
+            } else {
+                const float4 x = A.rec_x[slot + 1];
+                float3 Lk = make_float3(x.x, x.y, x.z);
+                if (have) {
+                    Lk.x += gprev.x * L.x;
+                    Lk.y += gprev.y * L.y;
+                    Lk.z += gprev.z * L.z;
+                }
+                const float4 g = A.rec_g[slot];
+                rp = A.rec_p[slot];
+                rw = A.rec_w[slot];
+                L = Lk;
+                gprev = make_float3(g.x, g.y, g.z);
+                have = true;
+                c = (0.2126f * Lk.x + 0.7152f * Lk.y + 0.0722f * Lk.z) * rp.w;
+                if (c > 0.f && isfinite(c)) {
+                    const uint64_t h = splitmix64(salt ^ ((uint64_t)pid << 8) ^ (uint64_t)k);
+                    emit = (float)(h >> 40) * 0x1p-24f < A.accept;
                 }
             }
+        }
+        const uint64_t me = __ballot(emit);
+        if (!me) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(A.ts_count, (uint32_t)__popcll(me));
+        base = __shfl(base, 0);
+        if (!emit) continue;
+        const uint32_t t = base + lanes_below(me);
+        if (t < A.ts_cap) {
+            A.ts[3 * (size_t)t] = make_float4(rp.x, rp.y, rp.z, c);
+            A.ts[3 * (size_t)t + 1] = make_float4(rn.x, rn.y, rn.z, rp.w);
+            A.ts[3 * (size_t)t + 2] = make_float4(rw.x, rw.y, rw.z, __uint_as_float((pid << 6) | (uint32_t)k));
         }
     }
 }
@@ -482,6 +640,7 @@ size_t carve(Queues& q, char* base, uint32_t cap) {
     q.so = (float4*)take((size_t)q.scap * 16);
     q.sd = (float4*)take((size_t)q.scap * 16);
     q.sc = (float4*)take((size_t)q.scap * 16);
+    q.sx = (float4*)take((size_t)q.scap * 16);
     q.cnt = (uint32_t*)take((4 * kMaxLevels + 1) * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
     q.cap = cap;
@@ -501,14 +660,38 @@ int alloc_queues(RenderBuffers& rb, uint32_t cap) {
     return used <= need ? TN_OK : set_error(TN_ERR_INVALID, "queue layout exceeds its allocation");
 }
 
+struct RecordBuffers {
+    float4* base = nullptr;
+    size_t slots = 0;
+};
+
+int alloc_records(RecordBuffers& rb, size_t slots) {
+    if (rb.base && rb.slots >= slots) return TN_OK;
+    if (rb.base) (void)hipFree(rb.base);
+    rb.base = nullptr;
+    rb.slots = 0;
+    TN_HIP(hipMalloc(&rb.base, slots * 5 * sizeof(float4)));
+    rb.slots = slots;
+    return TN_OK;
+}
+
 }  // namespace
 
 // Renders params->width x height x spp samples, adding the per-pixel SUM of
 // sample radiance into accum (device, RGBA).
 int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
                 const tn_render_params* P, float* accum, tn_render_stats* st, hipStream_t stream,
-                const float* ray_o = nullptr, const float* ray_d = nullptr, uint64_t nrays = 0) {
+                const float* ray_o = nullptr, const float* ray_d = nullptr, uint64_t nrays = 0,
+                tn_guide* guide = nullptr, uint32_t train_samples = 0) {
     const bool rays = ray_o != nullptr;
+    const bool guided = guide != nullptr;
+    const bool training = guided && train_samples > 0;
+    if ((P->flags & TN_RENDER_GUIDED) && !guided)
+        return set_error(TN_ERR_INVALID, "TN_RENDER_GUIDED needs a guide (tn_render_guided)");
+    if (training && P->mc_samples != 1)
+        return set_error(TN_ERR_INVALID, "training records need mc_samples == 1 (one path per sample)");
+    if (guided && guide->device != sc->device)
+        return set_error(TN_ERR_INVALID, "guide and scene live on different devices");
     if ((!rays && (P->width == 0 || P->height == 0 || P->pixel_samples < 1)) || P->max_depth < 0 ||
         P->mc_samples < 0)
         return set_error(TN_ERR_INVALID, "invalid render parameters");
@@ -563,6 +746,33 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
                                   lights[0].color[3]);
     }
     A.accum = (float4*)accum;
+    A.seed_pass = P->seed + (uint64_t)P->sample_offset * 0xD1B54A32D192ED03ull;
+    if (guided) {
+        A.gv = guide->view();
+        A.mis = guide->mis_guided;
+    }
+    static RecordBuffers recs;
+    if (training) {
+        A.rstride = (uint32_t)P->max_depth + 1;
+        rc = alloc_records(recs, (size_t)chunk * A.rstride);
+        if (rc) return rc;
+        const size_t ns = recs.slots;
+        A.rec_p = recs.base;
+        A.rec_n = recs.base + ns;
+        A.rec_w = recs.base + 2 * ns;
+        A.rec_g = recs.base + 3 * ns;
+        A.rec_x = recs.base + 4 * ns;
+        // every bounce of every path is a candidate; keep about kTrainSet of them
+        constexpr uint32_t kTrainSet = 1u << 21;
+        rc = guide_reserve_set(guide, kTrainSet, stream);
+        if (rc) return rc;
+        const uint64_t total_paths = rays ? nrays : (uint64_t)P->width * P->height * (uint32_t)P->pixel_samples;
+        const double candidates = (double)total_paths * std::max(1, P->max_depth);
+        A.accept = (float)std::min(1.0, 0.95 * kTrainSet / std::max(1.0, candidates));
+        A.ts = guide->ts;
+        A.ts_count = guide->ts_count;
+        A.ts_cap = guide->ts_cap;
+    }
     float* d_jit = nullptr;
     if (!rays && (P->flags & TN_RENDER_REF_JITTER)) {
         std::vector<float> jit = reference_jitter(P->width, A.spp);
@@ -581,6 +791,10 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     for (uint64_t first = 0; first < total; first += chunk) {
         const uint32_t cnt0 = (uint32_t)std::min<uint64_t>(chunk, total - first);
         k_chunk_init<<<1, 4 * kMaxLevels + 64, 0, stream>>>(rb.q.cnt, cnt0);
+        if (training) {
+            const uint32_t slots = cnt0 * A.rstride;
+            k_rec_clear<<<(slots + 255) / 256, 256, 0, stream>>>(A, slots);
+        }
         if (rays)
             k_raygen_rays<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, ray_o, ray_d,
                                                                                 first, cnt0);
@@ -590,14 +804,22 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
         for (int lv = 0; lv < levels; ++lv) {
             if (exh) k_trace<true><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
             else k_trace<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
-            k_shade<<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
+            if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, stream>>>(A, rb.q, lv, buf);
+            else k_shade<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
             if (exh) k_shadow<true><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv);
             else k_shadow<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv);
             buf ^= 1;
         }
         k_chunk_totals<<<1, 64, 0, stream>>>(rb.q.cnt, rb.q.totals, levels);
+        if (training)
+            k_collect<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(
+                A, cnt0, P->max_depth, A.seed_pass ^ (first * 0x9E3779B97F4A7C15ull));
     }
     TN_HIP(hipGetLastError());
+    if (training) {
+        rc = guide_train_from_set(guide, train_samples, stream);
+        if (rc) return rc;
+    }
     unsigned long long tot[2] = {0, 0};
     uint32_t overflow = 0;
     TN_HIP(hipMemcpyAsync(tot, rb.q.totals, 16, hipMemcpyDeviceToHost, stream));
@@ -629,6 +851,17 @@ int tn_render_accumulate(tn_scene* sc, const tn_camera* cam, const tn_light* lig
     if (!sc || !cam || !P || !accum || (nl && !lights))
         return set_error(TN_ERR_INVALID, "NULL argument");
     return render_into(sc, cam, lights, nl, P, accum, st, (hipStream_t)stream);
+}
+
+int tn_render_guided(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
+                     const tn_render_params* P, tn_guide* guide, uint32_t train_samples, float* accum,
+                     tn_render_stats* st, void* stream) {
+    if (!sc || !cam || !P || !guide || !accum || (nl && !lights))
+        return set_error(TN_ERR_INVALID, "NULL argument");
+    tn_render_params q = *P;
+    q.flags |= TN_RENDER_GUIDED;
+    return render_into(sc, cam, lights, nl, &q, accum, st, (hipStream_t)stream, nullptr, nullptr, 0, guide,
+                       train_samples);
 }
 
 int tn_trace(tn_scene* sc, const float* o3, const float* d3, uint32_t n, int32_t depth,

@@ -78,17 +78,27 @@ __device__ __forceinline__ float mixture_pdf(const Mixture& M, float3 w) {
     return q;
 }
 
-// Lobe by the CDF of pi (u0), vMF cosine by inverse CDF (u1), azimuth (u2).
+// Lobe by the CDF of pi (u0: the first lobe whose cumulative weight exceeds
+// u0, else the last), vMF cosine by inverse CDF (u1), azimuth (u2).  The
+// lobe is selected without dynamic indexing so M stays in registers.
 __device__ __forceinline__ float3 mixture_sample(const Mixture& M, float u0, float u1, float u2) {
-    int j = 0;
-    float c = M.pi[0];
-    while (j < kGLobes - 1 && u0 >= c) c += M.pi[++j];
-    const float k = M.kappa[j];
-    const float cw = fminf(1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k);
+    float c = 0.f, k = M.kappa[kGLobes - 1];
+    float mx = M.mu[kGLobes - 1][0], my = M.mu[kGLobes - 1][1], mz = M.mu[kGLobes - 1][2];
+    bool found = false;
+#pragma unroll
+    for (int q = 0; q < kGLobes - 1; ++q) {
+        c += M.pi[q];
+        const bool take = !found && u0 < c;
+        k = take ? M.kappa[q] : k;
+        mx = take ? M.mu[q][0] : mx;
+        my = take ? M.mu[q][1] : my;
+        mz = take ? M.mu[q][2] : mz;
+        found = found || take;
+    }
+    const float cw = fmaxf(-1.0f, fminf(1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k));
     const float sw = sqrtf(fmaxf(0.f, 1.f - cw * cw));
     float sp, cp;
     __sincosf(6.28318530717958647692f * u2, &sp, &cp);
-    const float mx = M.mu[j][0], my = M.mu[j][1], mz = M.mu[j][2];
     const float sign = copysignf(1.0f, mz);
     const float a = -1.0f / (sign + mz);
     const float b = mx * my * a;
@@ -140,67 +150,118 @@ __device__ __forceinline__ void guide_features(const GuideView& G, float3 p, flo
     row[79] = 0;
 }
 
-// One layer H^T = W X^T on the wave's 64 samples (4 tiles of 16) with
+// Y^T = W X^T on the wave's 64 samples (4 tiles of 16) with
 // mfma_f32_16x16x16bf16_1k.  A = W tile (lane: row 16mt + l%16, columns
-// 16kt + 4(l/16) .. +3), B = X^T tile read from the [sample][feature] LDS
-// rows (lane: sample 16nt + l%16, features 16kt + 4(l/16) .. +3), D (lane:
-// outputs 16mt + 4(l/16) + r, sample 16nt + l%16) is exactly the next
-// layer's B operand layout after bias, ReLU and BF16 rounding.
-template <int IN_TILES, int OUT_TILES, bool HIDDEN>
-__device__ __forceinline__ void guide_layer(const uint16_t* __restrict__ W, const float* __restrict__ bias,
-                                            int out_real, const uint16_t* in, int in_stride,
-                                            uint16_t* out_bf, int out_stride, float* out_f32) {
+// 16kt + 4(l/16) .. +3 of a row-major [16 OUT_TILES][16 IN_TILES] matrix),
+// B = X^T tile read from the [sample][feature] LDS rows (lane: sample
+// 16nt + l%16, features 16kt + 4(l/16) .. +3).  The accumulator element r of
+// lane l is output 16mt + 4(l/16) + r of sample 16nt + l%16, handed to
+// epi(output, sample, value).
+template <int IN_TILES, int OUT_TILES, class Epi>
+__device__ __forceinline__ void mfma_wx(const uint16_t* __restrict__ W, const uint16_t* in, int in_stride,
+                                        Epi epi) {
     const int l = threadIdx.x & 63;
     const int row = l & 15, grp = l >> 4;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         g_s4 bfrag[IN_TILES];
 #pragma unroll
-        for (int kt = 0; kt < IN_TILES; ++kt) {
-            const uint16_t* src = in + (16 * nt + row) * in_stride + 16 * kt + 4 * grp;
-            bfrag[kt] = *reinterpret_cast<const g_s4*>(src);
-        }
+        for (int kt = 0; kt < IN_TILES; ++kt)
+            bfrag[kt] = *reinterpret_cast<const g_s4*>(in + (16 * nt + row) * in_stride + 16 * kt + 4 * grp);
 #pragma unroll
         for (int mt = 0; mt < OUT_TILES; ++mt) {
             g_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kt = 0; kt < IN_TILES; ++kt) {
-                const g_s4 a = *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
+                const g_s4 a =
+                    *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfrag[kt], acc, 0, 0, 0);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int o = 16 * mt + 4 * grp + r;
-                const int smp = 16 * nt + row;
-                float v = acc[r] + (o < out_real ? bias[o] : 0.f);
-                if (HIDDEN) {
-                    out_bf[smp * out_stride + o] = to_bf16(fmaxf(v, 0.f));
-                } else if (o < out_real) {
-                    out_f32[smp * kGOut + o] = v;
-                }
-            }
+            for (int r = 0; r < 4; ++r) epi(16 * mt + 4 * grp + r, 16 * nt + row, acc[r]);
         }
     }
 }
 
-// Runs the MLP for the 64 lanes' samples of one wave (block of 64 threads).
-// Writes the 40 outputs of lane l's sample to out[l * 40 ..].  `lds` must
-// hold 64 * (kGStrideIn + kGStrideHid) uint16 values.
+// Hidden layer: bias, ReLU, BF16 into the next [sample][feature] rows.
+template <int IN_TILES>
+__device__ __forceinline__ void guide_hidden(const uint16_t* W, const float* bias, const uint16_t* in,
+                                             int in_stride, uint16_t* out, int out_stride) {
+    mfma_wx<IN_TILES, 4>(W, in, in_stride, [&](int o, int smp, float v) {
+        out[smp * out_stride + o] = to_bf16(fmaxf(v + bias[o], 0.f));
+    });
+}
+
+// Output layer: 40 FP32 outputs per sample into out[sample * 40 + o].
+__device__ __forceinline__ void guide_output(const uint16_t* W, const float* bias, const uint16_t* in,
+                                             int in_stride, float* out) {
+    mfma_wx<4, 3>(W, in, in_stride, [&](int o, int smp, float v) {
+        if (o < kGOut) out[smp * kGOut + o] = v + bias[o];
+    });
+}
+
+// Runs the MLP for the 64 lanes' samples of one wave (the block must be that
+// one wave: the layers are separated by block barriers).  Leaves the 40
+// outputs of lane l's sample at out[l * 40 ..] (out may alias the first
+// 64 * kGStrideIn uint16 of lds).  `lds` holds 64 * (kGStrideIn +
+// kGStrideHid) uint16 values.
 __device__ __forceinline__ void guide_forward_wave(const GuideView& G, float3 p, float3 n, bool active,
                                                    float* out, uint16_t* lds) {
     const int l = threadIdx.x & 63;
-    uint16_t* a = lds;                     // [64][kGStrideIn]
-    uint16_t* b = lds + 64 * kGStrideIn;   // [64][kGStrideHid]
+    uint16_t* a = lds;                    // [64][kGStrideIn]
+    uint16_t* b = lds + 64 * kGStrideIn;  // [64][kGStrideHid]
     guide_features(G, p, n, active, a + l * kGStrideIn);
     __syncthreads();
-    guide_layer<5, 4, true>(G.wbf + kGW1, G.theta + kGOffB1, 64, a, kGStrideIn, b, kGStrideHid, nullptr);
+    guide_hidden<5>(G.wbf + kGW1, G.theta + kGOffB1, a, kGStrideIn, b, kGStrideHid);
     __syncthreads();
-    guide_layer<4, 4, true>(G.wbf + kGW2, G.theta + kGOffB2, 64, b, kGStrideHid, a, kGStrideIn, nullptr);
+    guide_hidden<4>(G.wbf + kGW2, G.theta + kGOffB2, b, kGStrideHid, a, kGStrideIn);
     __syncthreads();
-    guide_layer<4, 4, true>(G.wbf + kGW3, G.theta + kGOffB3, 64, a, kGStrideIn, b, kGStrideHid, nullptr);
+    guide_hidden<4>(G.wbf + kGW3, G.theta + kGOffB3, a, kGStrideIn, b, kGStrideHid);
     __syncthreads();
-    guide_layer<4, 3, false>(G.wbf + kGW4, G.theta + kGOffB4, 40, b, kGStrideHid, nullptr, 0, out);
+    guide_output(G.wbf + kGW4, G.theta + kGOffB4, b, kGStrideHid, out);
     __syncthreads();
 }
 
+}  // namespace tn
+
+// Guide state behind tn_guide* (device buffers owned).
+struct tn_guide {
+    int device = 0;
+    float* theta = nullptr;   // FP32 master parameters
+    float* adam_m = nullptr;  // Adam moments
+    float* adam_v = nullptr;
+    uint16_t* wbf = nullptr;  // BF16 operand copies
+    float* partial = nullptr; // per-block gradient slices
+    uint32_t partial_blocks = 0;
+    float* dloss = nullptr;   // [0] loss of the last batch, [1] its sum of weights
+    float4* ts = nullptr;     // training set: 3 float4 per sample (p, c), (n, cos/pdf), (w, tag)
+    uint32_t ts_cap = 0;
+    uint32_t* ts_count = nullptr;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
+    uint64_t seed = 1;
+    uint32_t steps = 0;
+    float lr = 1e-3f;
+    uint32_t batch = 16384;
+    float mis_guided = 0.5f;
+    uint32_t last_samples = 0;
+
+    tn::GuideView view() const {
+        tn::GuideView v;
+        v.wbf = wbf;
+        v.theta = theta;
+        for (int k = 0; k < 3; ++k) {
+            v.bmin[k] = bmin[k];
+            const float ext = bmax[k] - bmin[k];
+            v.inv_ext[k] = ext > 0 ? 1.0f / ext : 0.f;
+        }
+        return v;
+    }
+};
+
+namespace tn {
+// Grows the training set to hold `cap` samples and clears it.
+int guide_reserve_set(tn_guide* g, uint32_t cap, hipStream_t stream);
+// train_samples / batch Adam steps on mini-batches drawn uniformly (with
+// replacement) from the training set.
+int guide_train_from_set(tn_guide* g, uint32_t train_samples, hipStream_t stream);
 }  // namespace tn
