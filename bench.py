@@ -38,6 +38,13 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    ap.add_argument("--no-guided", action="store_true",
+                    help="skip the guided-vs-plain time-to-relMSE measurement")
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the wavefront render throughput measurement")
+    ap.add_argument("--guided-ref-spp", type=int, default=2048)
+    ap.add_argument("--guided-width", type=int, default=512)
+    ap.add_argument("--render-spp", type=int, default=2)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one traversal launch plus a 1 GiB device copy (the calibration "
                          "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE), then exit")
@@ -185,7 +192,126 @@ def main():
                        f"oracle restatement of KDTreeIntersection::intersect (exhaustive), "
                        f"{ncores} threads, {csec:.1f} s; GPU ids identical on the sample"),
         }
+    if not args.no_render:
+        out["render"] = render_throughput(args)
+    if not args.no_guided:
+        out["guided"] = guided_vs_plain(args)
     print(json.dumps(out), flush=True)
+
+
+def render_throughput(args):
+    """Plain wavefront path tracing at scale: the displaced level-8 icosphere
+    (albedo 0.8) in the Cornell enclosure with a 0.2 x 0.2 emitter of
+    radiance 15; 1920 x 1080, max-depth 3, monte-carlo-samples 4, cosine
+    sampling + area-light next-event estimation.  Runs --render-spp pixel
+    samples (the rate does not depend on the sample count)."""
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE
+    sc = scenes.sphere_in_cornell(level=8)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    w, h = 1920, 1080
+    conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
+                          num_monte_carlo_samples=4, flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS, seed=1)
+    acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)  # warm-up (queue allocation)
+    conf.num_pixel_samples = args.render_spp
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc, sample_offset=1)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return {
+        "workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
+                     "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
+                     f"{args.render_spp} pixel samples timed"),
+        "closest_hit_mrays_s": round(st["rays"] / sec / 1e6, 1),
+        "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
+        "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 3),
+    }
+
+
+def guided_vs_plain(args):
+    """Occluded-light Cornell box (0.3 x 0.3 emitter of radiance 50 behind a
+    ceiling baffle, the level-8 displaced icosphere inside), 512 x 512,
+    max-depth 6, cosine sampling + area-light NEE: wall time until the
+    running image reaches relMSE 0.01 against a --guided-ref-spp reference,
+    for the plain tracer and for the learned-guided tracer (online training
+    included in its time), plus the MFMA rate of the guide kernels."""
+    import ctypes as C
+
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE
+    from turner_amd.guide import Guide, render_guided
+    from turner_amd.progressive import time_to_relmse
+    from turner_amd.relmse import reference_image
+    sc = scenes.occluded_light_cornell(level=8)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    w, depth = args.guided_width, 6
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    t0 = time.perf_counter()
+    ref = reference_image(tree, sc, w, depth, flags, spp=args.guided_ref_spp, seed=777)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    # warm-up of both paths (allocations), untimed
+    g = Guide.for_tree(tree, seed=1)
+    tiny = T.TracerConfig(width=w, max_recursion_depth=depth, num_pixel_samples=1, num_monte_carlo_samples=1,
+                          flags=flags)
+    scratch = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
+    T.render_accumulate(tree, sc["camera"], sc["lights"], tiny, scratch)
+    render_guided(tree, sc["camera"], sc["lights"], tiny, g, scratch, train_samples=1 << 14)
+    g.close()
+    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11)
+    g = Guide.for_tree(tree, seed=2)
+    guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12)
+    ginfo = g.info()
+    # MFMA rate of the guide kernels (HIP events on the launch stream)
+    dev = torch.device("cuda:0")
+    npts = 1 << 20
+    p = torch.rand((npts, 3), device=dev)
+    nrm = torch.nn.functional.normalize(torch.randn((npts, 3), device=dev), dim=1)
+    wd = torch.nn.functional.normalize(torch.randn((npts, 3), device=dev), dim=1)
+    cw = torch.rand(npts, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g.forward(p, nrm)
+    e0.record(stream)
+    for _ in range(5):
+        g.forward(p, nrm)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 5
+    nb = 1 << 16
+    g.train_step(p[:nb], nrm[:nb], wd[:nb], cw[:nb], want_loss=False)
+    e0.record(stream)
+    for _ in range(5):
+        g.train_step(p[:nb], nrm[:nb], wd[:nb], cw[:nb], want_loss=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    train_ms = e0.elapsed_time(e1) / 5
+    g.close()
+    fwd_flop = 31488.0  # 2 (78*64 + 64*64 + 64*64 + 64*40) per point
+    fwd_tf = fwd_flop * npts / (fwd_ms * 1e-3) / 1e12
+    train_tf = 3 * fwd_flop * nb / (train_ms * 1e-3) / 1e12
+    ratio = guided["seconds"] / plain["seconds"] if plain["reached"] and guided["reached"] else None
+    return {
+        "workload": (f"occluded-light Cornell box + level-8 icosphere, {w}x{w}, max-depth 6, cosine + "
+                     f"area-light NEE, relMSE target 0.01 vs a {args.guided_ref_spp}-spp reference "
+                     f"({ref_s:.1f} s)"),
+        "plain": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items() if k != "history"},
+        "guided": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in guided.items() if k != "history"},
+        "guided_time_over_plain": None if ratio is None else round(ratio, 4),
+        "guide_steps": ginfo["steps"], "guide_last_loss": round(ginfo["loss"], 4),
+        "mfma": {"forward_tflops": round(fwd_tf, 2), "train_tflops": round(train_tf, 2),
+                 "peak_bf16_tflops": 2500.0, "forward_frac": round(fwd_tf / 2500.0, 5),
+                 "train_frac": round(train_tf / 2500.0, 5),
+                 "forward_ms_per_2p20_points": round(fwd_ms, 4), "train_ms_per_2p16_batch": round(train_ms, 4)},
+    }
 
 
 if __name__ == "__main__":

@@ -198,7 +198,7 @@ int tn_render_accumulate(tn_scene* scene, const tn_camera* cam, const tn_light* 
 
 /*
  * Batched twin of trace(ray, tree_intersection, lights, depth, conf)
- * (pathtracer.cpp:14-102, trace.h:18-20): traces n caller rays (device
+ * (pathtracer.cpp:14-102, trace.h:23-25): traces n caller rays (device
  * arrays o3, d3) starting at recursion level `depth` and ADDS each ray's
  * radiance (RGBA) to rgba[4 i .. 4 i + 3] (device).  The estimator, flags,
  * background, m (mc_samples) and max_depth come from params (width, height,

@@ -4,7 +4,7 @@
 //
 //   KDTree(Triangles)                      lib/kdtree.h:187-264
 //   KDTreeIntersection::intersect(...)     lib/kdtree.h:270-309
-//   trace(ray, intersection, lights, depth, conf)   trace.h:18-20
+//   trace(ray, intersection, lights, depth, conf)   trace.h:23-25
 //   TracerConfig                           config.h:103-153
 //
 // Errors are reported as turner_amd::Error exceptions carrying the C ABI
@@ -134,7 +134,7 @@ struct TracerConfig {  // config.h:45-60 + 103-153, pathtracer USAGE defaults
     std::string tree_cache;
 };
 
-// trace.h:18-20: radiance of one ray from recursion level `depth`.
+// trace.h:23-25: radiance of one ray from recursion level `depth`.
 Color trace(const Ray& ray, KDTreeIntersection& ti, const std::vector<Light>& lights, int depth,
             const TracerConfig& conf);
 // Batched twin: radiance of many rays.

@@ -1,0 +1,59 @@
+"""Progressive rendering until a relMSE target: the plain cosine tracer and
+the learned-guided tracer, timed the same way.
+
+Both add passes of `spp_per_pass` samples per pixel (fresh random streams per
+pass through sample_offset) into one FP32 accumulator and stop when the
+running mean reaches the target relMSE against a reference image.  Only the
+render (and, for the guided tracer, the online training that follows each
+training pass) is timed; evaluating the error is not.  The guided tracer
+trains on the radiance samples of its first `train_passes` passes
+(train_samples per pass) and keeps every pass in the image: one-sample MIS
+with the exact mixture pdf keeps each pass unbiased whatever the state of
+the network.
+"""
+import time
+
+import torch
+
+from . import TracerConfig, render_accumulate
+from .relmse import relmse
+
+
+def _conf(width, height, depth, flags, spp, seed):
+    return TracerConfig(width=width, aspect=width / height, max_recursion_depth=depth, num_pixel_samples=spp,
+                        num_monte_carlo_samples=1, flags=flags, seed=seed)
+
+
+def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=4,
+                   train_passes=8, train_samples=1 << 18, train_spp=1, max_seconds=60.0, seed=1,
+                   max_spp=1 << 16):
+    """Returns a dict: seconds (timed render + training), spp, relmse,
+    reached (bool), rays, history [(seconds, spp, relmse)]."""
+    from .guide import render_guided
+    h, w = ref.shape[0], ref.shape[1]
+    acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    spent, spp, rays, hist = 0.0, 0, 0, []
+    err = float("inf")
+    k = 0
+    while spent < max_seconds and spp < max_spp:
+        training = guide is not None and k < train_passes
+        n = train_spp if training else spp_per_pass
+        conf = _conf(w, h, depth, flags, n, seed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if guide is None:
+            st = render_accumulate(tree, scene["camera"], scene["lights"], conf, acc, sample_offset=spp)
+        else:
+            st = render_guided(tree, scene["camera"], scene["lights"], conf, guide, acc,
+                               train_samples=train_samples if training else 0, sample_offset=spp)
+        torch.cuda.synchronize()
+        spent += time.perf_counter() - t0
+        spp += n
+        rays += st["rays"] + st["shadow_rays"]
+        k += 1
+        err = relmse(acc / spp, ref)
+        hist.append((spent, spp, err))
+        if err <= target:
+            break
+    return {"seconds": spent, "spp": spp, "relmse": err, "reached": err <= target, "rays": rays,
+            "history": hist}
