@@ -12,7 +12,7 @@ tests/test_guide_oracle.py):
 * against the unrounded FP64 model (the BF16 model error): outputs within
   0.02, gradient within 0.1 relative L2 and cosine >= 0.995;
 * Adam: the step the kernel takes equals numpy's FP64 Adam on the kernel's
-  own gradient within 1e-5 relative;
+  own gradient within 2e-4 relative (plus half an FP32 ulp of the parameter);
 * mixture pdf and sampler against the oracle on the kernel's own outputs:
   pdf within 2e-4 relative, directions within 1e-3 for 99.9%; the pdf
   integrates to 1 over the sphere within 1%;
@@ -166,7 +166,9 @@ def test_adam_step_matches_numpy(guide):
         new = guide.params().astype(np.float64)
         ref, m, v = OG.adam(cur, g, m, v, t)
         d_gpu, d_ref = new - cur, ref - cur
-        np.testing.assert_allclose(d_gpu, d_ref, rtol=2e-4, atol=1e-9)
+        # the kernel's FP32 parameters round each update by half an ulp of theta
+        bound = 2e-4 * np.abs(d_ref) + 6e-8 * np.abs(ref) + 1e-9
+        assert np.all(np.abs(d_gpu - d_ref) <= bound), np.max(np.abs(d_gpu - d_ref) / bound)
         steps.append(np.abs(d_gpu).max())
         cur = new
     assert max(steps) <= 3.2e-3  # |step| <= lr (1 - b1) / sqrt(1 - b2)
@@ -302,9 +304,28 @@ def test_training_records_identities(gpu):
     g.close()
 
 
-def _zcheck(gpu_mean, n_gpu, cpu_mean, cpu_sq, n_cpu, what):
-    from tests.test_gpu_render import zcheck
-    zcheck(gpu_mean, n_gpu, cpu_mean, cpu_sq, n_cpu, what)
+def _zcheck_passes(passes, cpu_mean, cpu_sq, n_cpu, what):
+    """passes: K independent GPU images (per-pixel means of equal sample
+    counts).  The guided estimator's variance differs from the cosine
+    estimator's, so sigma combines the spread of the GPU passes with the
+    oracle's per-sample variance."""
+    passes = np.asarray(passes, np.float64)
+    k = len(passes)
+    gm = passes.mean(0)
+    gvar = passes.var(0, ddof=1) / k
+    cvar = np.maximum(cpu_sq / n_cpu - cpu_mean.astype(np.float64) ** 2, 0.0) / n_cpu
+    sig = np.sqrt(gvar + cvar)[..., :3]
+    diff = (gm - cpu_mean)[..., :3]
+    ok = sig > 1e-9
+    assert np.all(np.abs(diff[~ok]) <= 1e-5 + 1e-5 * np.abs(cpu_mean[..., :3][~ok]))
+    z = diff[ok] / sig[ok]
+    frac = np.mean(np.abs(z) > 3)
+    print(f"[{what}] pixel-channels {z.size}, |z|>3: {frac:.4f}, mean z {z.mean():+.3f}, "
+          f"std z {z.std():.3f}", flush=True)
+    # K = 16 passes: sigma is itself estimated, so z follows a Student-like law
+    assert frac <= 0.02
+    assert abs(z.mean()) < 0.25
+    assert 0.7 < z.std() < 1.4
 
 
 def test_guided_render_is_unbiased(gpu):
@@ -316,27 +337,33 @@ def test_guided_render_is_unbiased(gpu):
     sc = scenes.sphere_in_cornell(level=3)
     tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
     ot = O.Tree(sc["verts"])
-    w, n_cpu, n_gpu = 24, 512, 512
+    w, n_cpu = 24, 1024
     flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
     ref, sq, _, _ = O.render(sc, ot, w, w, 3, 1, n_cpu, flags=O.FLAG_COSINE | O.FLAG_AREA_LIGHTS, seed=3,
                              want_sq=True)
     g = Guide.for_tree(tree, seed=7, batch=4096)
-    conf = T.TracerConfig(width=w, max_recursion_depth=3, num_pixel_samples=n_gpu, num_monte_carlo_samples=1,
-                          flags=flags, seed=17)
-    acc = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
-    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc)
-    _zcheck(acc.cpu().numpy() / n_gpu, n_gpu, ref, sq, n_cpu, "guided, untrained")
+    passes, per = 16, 64
+
+    def guided_passes(seed):
+        out = []
+        for k in range(passes):
+            conf = T.TracerConfig(width=w, max_recursion_depth=3, num_pixel_samples=per,
+                                  num_monte_carlo_samples=1, flags=flags, seed=seed)
+            acc = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
+            render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, sample_offset=k * per)
+            out.append(acc.cpu().numpy() / per)
+        return out
+
+    _zcheck_passes(guided_passes(17), ref, sq, n_cpu, "guided, untrained")
     # train on a few passes, then render with the trained guide
     tconf = T.TracerConfig(width=64, max_recursion_depth=3, num_pixel_samples=4, num_monte_carlo_samples=1,
                            flags=flags, seed=19)
     tacc = torch.zeros((64, 64, 4), dtype=torch.float32, device="cuda")
     for k in range(8):
-        render_guided(tree, sc["camera"], sc["lights"], tconf, g, tacc, train_samples=1 << 16, sample_offset=k * 4)
+        render_guided(tree, sc["camera"], sc["lights"], tconf, g, tacc, train_samples=1 << 16,
+                      sample_offset=k * 4)
     assert g.info()["steps"] >= 8
-    acc.zero_()
-    conf.seed = 23
-    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc)
-    _zcheck(acc.cpu().numpy() / n_gpu, n_gpu, ref, sq, n_cpu, "guided, trained")
+    _zcheck_passes(guided_passes(23), ref, sq, n_cpu, "guided, trained")
     g.close()
 
 
