@@ -42,7 +42,7 @@ template <int SCHED, bool EXH, bool MT, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __restrict__ o3,
                                                  const float* __restrict__ d3,
                                                  const float* __restrict__ tmax, uint32_t n,
-                                                 uint32_t* __restrict__ out_id,
+                                                 uint32_t chunk, uint32_t* __restrict__ out_id,
                                                  float* __restrict__ out_r,
                                                  float* __restrict__ out_s,
                                                  float* __restrict__ out_t,
@@ -59,8 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
     const uint32_t per_xcd = gridDim.x / kXcds;
     const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
     const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
-    uint32_t next = wave * (uint32_t)kChunk;
-    const uint32_t end = next < n ? min(next + (uint32_t)kChunk, n) : next;
+    uint32_t next = wave * chunk;
+    const uint32_t end = next < n ? min(next + chunk, n) : next;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t ray = kNone;
     Trav T;
@@ -130,7 +130,7 @@ constexpr int kCoopWavesPerEU = 7;  // measured best (5: 964, 6: 964, 7: 1036, 8
 template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rays_coop(
     SceneView sc, const float* __restrict__ o3, const float* __restrict__ d3,
-    const float* __restrict__ tmax, uint32_t n, uint32_t* __restrict__ out_id,
+    const float* __restrict__ tmax, uint32_t n, uint32_t chunk, uint32_t* __restrict__ out_id,
     float* __restrict__ out_r, float* __restrict__ out_s, float* __restrict__ out_t,
     uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t per_xcd = gridDim.x / kXcds;
     const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
     const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
-    const uint32_t first = wave * (uint32_t)kChunk;
+    const uint32_t first = wave * chunk;
     bool given = false;
     uint32_t nrays = 0;
     TraversalCounters cnt;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (given || first >= n) return false;
             given = true;
             b = first;
-            e = min(first + (uint32_t)kChunk, n);
+            e = min(first + chunk, n);
             return true;
         },
         [&](uint32_t k, float3& o, float3& d, float& tm) {
@@ -324,14 +324,17 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     if (n == 0) return TN_OK;
     if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
     const SceneView v = sc->view();
-    const uint32_t waves = (n + kChunk - 1) / kChunk;
+    // rays per wave: up to kChunk, fewer when the batch would not give every
+    // resident wave (7 per SIMD) work
+    const uint32_t chunk = rays_per_wave(n, sc->cus);
+    const uint32_t waves = (n + chunk - 1) / chunk;
     const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
     const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
                cnt = flags & TN_ISECT_COUNT, any = occ != nullptr,
                tick = flags & TN_ISECT_TICK, leaf = flags & TN_ISECT_LEAF;
     auto go = [&](auto kern) {
-        kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, id, r, s, t, occ, sc->d_counters,
+        kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, chunk, id, r, s, t, occ, sc->d_counters,
                                          sc->d_status);
     };
     const uint32_t wpe = (flags >> 24) & 15u;  // occupancy variant (tuning), 0 = default
@@ -416,6 +419,7 @@ int tn_scene_create(const float* verts9, const float* normals9, const float* alb
     std::unique_ptr<tn_scene> sc(new (std::nothrow) tn_scene);
     if (!sc) return set_error(TN_ERR_OOM, "host allocation failed");
     int rc = check_device(&sc->device);
+    if (!rc) (void)hipDeviceGetAttribute(&sc->cus, hipDeviceAttributeMultiprocessorCount, sc->device);
     if (rc) return rc;
     try {
         if ((rc = prepare_triangles(sc.get(), verts9, n))) return rc;
@@ -447,6 +451,7 @@ int tn_scene_create_with_tree(const float* verts9, const float* normals9, const 
     std::unique_ptr<tn_scene> sc(new (std::nothrow) tn_scene);
     if (!sc) return set_error(TN_ERR_OOM, "host allocation failed");
     int rc = check_device(&sc->device);
+    if (!rc) (void)hipDeviceGetAttribute(&sc->cus, hipDeviceAttributeMultiprocessorCount, sc->device);
     if (rc) return rc;
     try {
         if ((rc = prepare_triangles(sc.get(), verts9, n))) return rc;

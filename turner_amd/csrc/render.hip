@@ -223,16 +223,26 @@ __global__ __launch_bounds__(kRBlock) void k_raygen_rays(RenderArgs A, Queues Q,
 }
 
 // Per-wave dynamic chunks of a device-counted queue (one atomic per chunk).
+// The chunk shrinks with the queue (rays_per_wave over a quarter of the
+// grid's waves, so a wave takes about four chunks) to keep every wave busy
+// on short queues.
 struct DynamicChunks {
     uint32_t n;
     uint32_t* work;
+    uint32_t size;
+    __device__ DynamicChunks(uint32_t n_, uint32_t* work_) : n(n_), work(work_) {
+        const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+        uint32_t c = (n + 4 * waves - 1) / (4 * waves);
+        c = (c + 63u) & ~63u;
+        size = c < 64u ? 64u : (c > (uint32_t)kChunk ? (uint32_t)kChunk : c);
+    }
     __device__ bool operator()(uint32_t& b, uint32_t& e) const {
         uint32_t base = 0;
-        if ((threadIdx.x & 63u) == 0) base = atomicAdd(work, (uint32_t)kChunk);
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(work, size);
         base = __shfl(base, 0, 64);
         if (base >= n) return false;
         b = base;
-        e = min(base + (uint32_t)kChunk, n);
+        e = min(base + size, n);
         return true;
     }
 };

@@ -16,6 +16,7 @@
 struct tn_scene {
     uint32_t ntris = 0;
     int device = 0;
+    int cus = 256;  // compute units of the device
     tn::HostTree tree;
     std::vector<float> pre16;
     uint32_t nemit = 0;
@@ -61,6 +62,18 @@ constexpr int kBlock = 256;
 constexpr int kStack = 8;
 constexpr int kChunk = 512;  // rays per wave in the refill kernels
 constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, one L2 each
+constexpr uint32_t kResidentWavesPerSimd = 7;
+
+// Rays per wave for a batch of n: kChunk when the batch fills every resident
+// wave slot (cus x 4 SIMDs x 7), otherwise fewer (a multiple of 64, at least
+// 64) so that the batch still spreads over the whole chip -- a traversal is
+// latency-bound, and a wave alone on its SIMD cannot hide it.
+__host__ __device__ inline uint32_t rays_per_wave(uint32_t n, uint32_t cus) {
+    const uint32_t slots = (cus ? cus : 256u) * 4u * kResidentWavesPerSimd;
+    uint32_t c = (uint32_t)(((uint64_t)n + slots - 1) / slots);
+    c = (c + 63u) & ~63u;
+    return c < 64u ? 64u : (c > (uint32_t)kChunk ? (uint32_t)kChunk : c);
+}
 
 // Traversal over n rays (device arrays): closest hit into id/r/s/t, or with
 // occ != NULL visibility against tmax into occ.

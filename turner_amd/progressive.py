@@ -24,11 +24,12 @@ def _conf(width, height, depth, flags, spp, seed):
                         num_monte_carlo_samples=1, flags=flags, seed=seed)
 
 
-def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=4,
-                   train_passes=8, train_samples=1 << 18, train_spp=1, max_seconds=60.0, seed=1,
-                   max_spp=1 << 16):
+def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=16,
+                   train_passes=16, train_samples=1 << 18, train_spp=1, max_seconds=60.0, seed=1,
+                   max_spp=1 << 16, ref_var=None):
     """Returns a dict: seconds (timed render + training), spp, relmse,
-    reached (bool), rays, history [(seconds, spp, relmse)]."""
+    reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
+    reference's per-pixel variance (relmse subtracts it)."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -51,7 +52,7 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
         spp += n
         rays += st["rays"] + st["shadow_rays"]
         k += 1
-        err = relmse(acc / spp, ref)
+        err = relmse(acc / spp, ref, ref_var)
         hist.append((spent, spp, err))
         if err <= target:
             break
