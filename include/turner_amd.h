@@ -155,6 +155,8 @@ typedef struct {
 #define TN_RENDER_REF_JITTER 4u  /* pixel jitter of main.cpp:201-206 (xorshift64* seeded 42 per row) */
 #define TN_RENDER_GUIDED 8u      /* learned guiding proposal (needs a tn_guide) */
 #define TN_RENDER_EXHAUSTIVE 16u /* reference traversal order without early exit */
+#define TN_RENDER_GRAPH 32u      /* capture the per-chunk launch sequence once in a hipGraph, replay per chunk */
+#define TN_RENDER_AUTOTUNE 64u   /* batches: chunk size and grid picked by timing probes (cached) */
 
 typedef struct {
     uint32_t width, height;
@@ -285,6 +287,42 @@ long tn_guide_get_train_set(tn_guide* g, float* out, size_t cap);
 int tn_render_guided(tn_scene* scene, const tn_camera* cam, const tn_light* lights,
                      uint32_t num_lights, const tn_render_params* params, tn_guide* guide,
                      uint32_t train_samples, float* accum, tn_render_stats* stats, void* stream);
+
+/*
+ * Batched heterogeneous rendering (no reference counterpart: the reference
+ * renders one scene per process).  tn_batch_create packs the trees and
+ * shading data of n scenes into one forest (the scenes may be destroyed
+ * afterwards); tn_batch_render renders one job per scene -- camera, at most
+ * one point light, image size, pixel samples, background -- through one
+ * wavefront launch sequence per chunk of samples, where chunks cut across
+ * scenes and every ray finds its scene from its pixel.  accum (device)
+ * receives the per-pixel SUMS of all images back to back: job j's image starts
+ * at float offset 4 * sum_{i<j} width_i * height_i.  From params only
+ * max_depth, mc_samples, flags, seed, chunk_paths and sample_offset are used.
+ */
+typedef struct tn_batch tn_batch;
+typedef struct {
+    tn_camera cam;
+    tn_light light;
+    uint32_t num_lights; /* 0 or 1 */
+    uint32_t width, height;
+    int32_t pixel_samples;
+    float bg[4];
+} tn_batch_job;
+typedef struct {
+    uint32_t chunk_paths;   /* primary samples per chunk */
+    uint32_t blocks_per_cu; /* grid of the persistent trace / shade / shadow kernels */
+    uint32_t graph;         /* chunk sequence replayed from a hipGraph */
+    uint32_t candidates;    /* launch shapes timed */
+    double probe_ms;        /* time spent tuning */
+    double best_rate;       /* samples per ms of the chosen shape on the probe */
+} tn_batch_tuning;
+int tn_batch_create(tn_scene* const* scenes, uint32_t num_scenes, tn_batch** out);
+int tn_batch_destroy(tn_batch* batch);
+int tn_batch_render(tn_batch* batch, const tn_batch_job* jobs, const tn_render_params* params,
+                    float* accum, tn_render_stats* stats, void* stream);
+/* The launch shape the autotuner chose for the last TN_RENDER_AUTOTUNE render. */
+int tn_batch_get_tuning(const tn_batch* batch, tn_batch_tuning* out);
 
 const char* tn_last_error(void);
 /* Device name and compute-unit count of the current HIP device. */

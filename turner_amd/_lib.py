@@ -32,6 +32,8 @@ TN_RENDER_AREA_LIGHTS = 2
 TN_RENDER_REF_JITTER = 4
 TN_RENDER_GUIDED = 8
 TN_RENDER_EXHAUSTIVE = 16
+TN_RENDER_GRAPH = 32
+TN_RENDER_AUTOTUNE = 64
 
 
 class TurnerError(RuntimeError):
@@ -63,6 +65,16 @@ class RenderParams(C.Structure):
                 ("inverse_gamma", C.c_float), ("gamma_enabled", C.c_int32),
                 ("chunk_paths", C.c_uint32), ("sample_offset", C.c_uint32),
                 ("reserved", C.c_uint32)]
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("cam", Camera), ("light", Light), ("num_lights", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("pixel_samples", C.c_int32), ("bg", C.c_float * 4)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("chunk_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("graph", C.c_uint32),
+                ("candidates", C.c_uint32), ("probe_ms", C.c_double), ("best_rate", C.c_double)]
 
 
 class RenderStats(C.Structure):
@@ -116,6 +128,10 @@ SIGNATURES = {
     "tn_render_guided": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(Light), _U32,
                                    C.POINTER(RenderParams), _P, _U32, _P,
                                    C.POINTER(RenderStats), _P]),
+    "tn_batch_create": (C.c_int, [_P, _U32, C.POINTER(_P)]),
+    "tn_batch_destroy": (C.c_int, [_P]),
+    "tn_batch_render": (C.c_int, [_P, _P, C.POINTER(RenderParams), _P, C.POINTER(RenderStats), _P]),
+    "tn_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "tn_last_error": (C.c_char_p, []),
     "tn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
 }

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             e = min(first + chunk, n);
             return true;
         },
-        [&](uint32_t k, float3& o, float3& d, float& tm) {
+        [&](uint32_t k, float3& o, float3& d, float& tm, uint32_t&) {
             o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
             d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
             if (ANY) tm = tmax[k];

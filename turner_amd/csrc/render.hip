@@ -56,6 +56,7 @@ namespace {
 
 constexpr int kRBlock = 256;
 constexpr uint32_t kMaxLevels = 64;
+constexpr uint64_t kSceneSeedStep = 0x9E3779B97F4A7C15ull;
 
 struct Queues {
     float4* o[2];   // o.xyz, pixel (bits)
@@ -75,6 +76,28 @@ struct Queues {
     unsigned long long* totals;  // [0] segments, [1] shadow rays
 };
 
+// Per-scene parameters of a render: one entry per scene of a batch (one for
+// a single-scene render).  Global sample j of the call belongs to the last
+// scene whose sample_begin <= j; its pixel p to the last whose pixel_begin <= p.
+struct SceneParams {
+    float4 cam_pos;
+    float cam_m[9];
+    float cam_dx, cam_dy;
+    uint32_t width, height, spp, nlights;
+    float4 light_pos, light_col, bg;
+    unsigned long long sample_begin;  // first global sample index
+    uint32_t pixel_begin;             // first accumulator pixel
+    uint32_t emit_begin, nemit;       // emitter range (emit_ids / emit_cdf)
+    float emit_area;
+};
+
+// Chunk bounds kept on the device, so that one captured launch sequence
+// serves every chunk: k_chunk_begin advances them.
+struct ChunkState {
+    unsigned long long next, total, first;
+    uint32_t chunk, count;
+};
+
 struct RenderArgs {
     SceneView sv;
     const float* normals;  // 9 per triangle
@@ -82,20 +105,14 @@ struct RenderArgs {
     const float4* emission;
     const float4* trirec;  // device triangle records (geometric normal in rec[1])
     const uint32_t* emit_ids;
-    const float* emit_cdf;
-    uint32_t nemit;
-    float emit_area;
-    float4 cam_pos;
-    float cam_m[9];
-    float cam_dx, cam_dy;
-    uint32_t width, height, spp;
+    const float* emit_cdf;  // per scene: inclusive prefix of its emitters' areas
+    const SceneParams* sp;
+    uint32_t nscenes;
+    ChunkState* cs;
     int32_t max_depth, mc;
     uint32_t flags;
     uint64_t seed;
-    float4 bg;
-    uint32_t nlights;
-    float4 light_pos, light_col;
-    const float* jitter;  // 2 * width * spp (REF_JITTER) or null
+    const float* jitter;  // 2 * width * spp (REF_JITTER, single scene) or null
     float4* accum;
     uint64_t seed_pass;  // seed of this call's sample range (sample_offset)
     // guided sampling
@@ -113,6 +130,28 @@ struct RenderArgs {
     uint32_t ts_cap;
     float accept;  // probability that a record enters the training set
 };
+
+__device__ __forceinline__ uint32_t scene_of_pixel(const RenderArgs& A, uint32_t pixel) {
+    if (A.nscenes <= 1) return 0;
+    uint32_t lo = 0, hi = A.nscenes - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (A.sp[mid].pixel_begin <= pixel) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_sample(const RenderArgs& A, unsigned long long j) {
+    if (A.nscenes <= 1) return 0;
+    uint32_t lo = 0, hi = A.nscenes - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (A.sp[mid].sample_begin <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
@@ -154,9 +193,20 @@ __device__ __forceinline__ void accumulate(float4* acc, uint32_t pixel, float4 c
     atomicAdd(p + 3, c.w);
 }
 
-__global__ void k_chunk_init(uint32_t* cnt, uint32_t n0) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 4 * kMaxLevels + 1) cnt[i] = (i == 0) ? n0 : 0u;
+// Starts the next chunk: [first, first + count) of the call's samples; zeroes
+// the level counters and sets the level-0 count.
+__global__ void k_chunk_begin(uint32_t* cnt, ChunkState* cs) {
+    const uint32_t i = threadIdx.x;
+    const unsigned long long first = cs->next;
+    const unsigned long long left = cs->total > first ? cs->total - first : 0ull;
+    const uint32_t count = (uint32_t)(left < cs->chunk ? left : cs->chunk);
+    if (i < 4 * kMaxLevels + 1) cnt[i] = (i == 0) ? count : 0u;
+    __syncthreads();
+    if (i == 0) {
+        cs->first = first;
+        cs->count = count;
+        cs->next = first + count;
+    }
 }
 
 __global__ void k_zero_totals(unsigned long long* totals) {
@@ -175,19 +225,23 @@ __global__ void k_chunk_totals(const uint32_t* cnt, unsigned long long* totals, 
     }
 }
 
-// Primary samples: j = (y * width + x) * spp + i  (main.cpp:193-215 order).
-__global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q, uint64_t first,
-                                                    uint32_t count) {
+// Primary samples: within a scene, j = (y * width + x) * spp + i
+// (main.cpp:193-215 order); scenes follow each other in the sample space.
+__global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const uint64_t j = first + i;
-    const uint32_t sample = (uint32_t)(j % A.spp);
-    const uint64_t pix = j / A.spp;
-    const uint32_t x = (uint32_t)(pix % A.width), y = (uint32_t)(pix / A.width);
-    uint4 rng = rng_seed(A.seed_pass, j);
+    if (i >= A.cs->count) return;
+    const unsigned long long j = A.cs->first + i;
+    const uint32_t si = scene_of_sample(A, j);
+    const SceneParams& S = A.sp[si];
+    const unsigned long long local = j - S.sample_begin;
+    const uint32_t sample = (uint32_t)(local % S.spp);
+    const uint64_t pix = local / S.spp;
+    const uint32_t x = (uint32_t)(pix % S.width), y = (uint32_t)(pix / S.width);
+    // scene s of a batch draws the streams of a single render seeded seed + s * kSceneSeedStep
+    uint4 rng = rng_seed(A.seed_pass + si * kSceneSeedStep, local);
     float dx, dy;
     if (A.jitter) {
-        const size_t k = 2 * ((size_t)x * A.spp + sample);
+        const size_t k = 2 * ((size_t)x * S.spp + sample);
         dx = A.jitter[k];
         dy = A.jitter[k + 1];
     } else {
@@ -196,26 +250,25 @@ __global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q, uint
     }
     // lib/types.h:119-123
     const float px = (float)x + dx, py = (float)y + dy;
-    const float w = (float)A.width, h = (float)A.height;
-    const float cx = -A.cam_dx * (1 - 2 * px / w);
-    const float cy = A.cam_dy * (1 - 2 * py / h);
+    const float w = (float)S.width, h = (float)S.height;
+    const float cx = -S.cam_dx * (1 - 2 * px / w);
+    const float cy = S.cam_dy * (1 - 2 * py / h);
     const float cz = -1;
-    const float* m = A.cam_m;
+    const float* m = S.cam_m;
     const float3 d = make_float3(m[0] * cx + m[1] * cy + m[2] * cz, m[3] * cx + m[4] * cy + m[5] * cz,
                                  m[6] * cx + m[7] * cy + m[8] * cz);
-    Q.o[0][i] = make_float4(A.cam_pos.x, A.cam_pos.y, A.cam_pos.z, __uint_as_float((uint32_t)pix));
+    Q.o[0][i] = make_float4(S.cam_pos.x, S.cam_pos.y, S.cam_pos.z, __uint_as_float(S.pixel_begin + (uint32_t)pix));
     Q.d[0][i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));  // .w: path within the chunk
     Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
     Q.rng[0][i] = rng;
 }
 
-// Caller rays (tn_trace): ray i accumulates into accum[i]; seeds from (seed, i).
+// Caller rays (tn_trace): ray j accumulates into accum[j]; seeds from (seed, j).
 __global__ __launch_bounds__(kRBlock) void k_raygen_rays(RenderArgs A, Queues Q, const float* o3,
-                                                         const float* d3, uint64_t first,
-                                                         uint32_t count) {
+                                                         const float* d3) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const uint64_t j = first + i;
+    if (i >= A.cs->count) return;
+    const uint64_t j = A.cs->first + i;
     Q.o[0][i] = make_float4(o3[3 * j], o3[3 * j + 1], o3[3 * j + 2], __uint_as_float((uint32_t)j));
     Q.d[0][i] = make_float4(d3[3 * j], d3[3 * j + 1], d3[3 * j + 2], __uint_as_float(i));
     Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -260,10 +313,11 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     coop_loop<kStack, EXH, false, false, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 2 * kMaxLevels + level},
-        [&](uint32_t k, float3& o, float3& d, float&) {
+        [&](uint32_t k, float3& o, float3& d, float&, uint32_t& scene) {
             const float4 a = qo[k], b = qd[k];
             o = make_float3(a.x, a.y, a.z);
             d = make_float3(b.x, b.y, b.z);
+            scene = scene_of_pixel(A, __float_as_uint(a.w));
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
             const bool h = traversed && T.best_id != kInvalidId;
@@ -283,11 +337,12 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     coop_loop<kStack, EXH, false, true, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
-        [&](uint32_t k, float3& o, float3& d, float& tmax) {
+        [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
             const float4 a = Q.so[k], b = Q.sd[k];
             o = make_float3(a.x, a.y, a.z);
             d = make_float3(b.x, b.y, b.z);
             tmax = b.w;
+            scene = scene_of_pixel(A, __float_as_uint(a.w));
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
             if (!traversed || T.best_id == kInvalidId) {
@@ -335,7 +390,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
     const bool spawn = level < A.max_depth && A.mc > 0;
     const uint32_t m = (uint32_t)max(A.mc, 0);
     const bool cosine = A.flags & TN_RENDER_COSINE;
-    const bool area = (A.flags & TN_RENDER_AREA_LIGHTS) && A.nemit > 0;
+    const bool area_flag = (A.flags & TN_RENDER_AREA_LIGHTS) != 0;
     const bool records = GUIDED && A.rec_p != nullptr;
     const float inv_pi = 0.318309886183790671538f;
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
@@ -354,10 +409,11 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
             rng = Q.rng[buf][i];
             pixel = __float_as_uint(qo.w);
             slot = __float_as_uint(qd.w) * A.rstride + (uint32_t)level;
+            const SceneParams& S = A.sp[scene_of_pixel(A, pixel)];
             const uint4 h = Q.hit[i];
             if (h.x == kInvalidId) {
-                accumulate(A.accum, pixel, mul4(w, A.bg));  // pathtracer.cpp:26-28
-                if (records) A.rec_x[slot] = A.bg;
+                accumulate(A.accum, pixel, mul4(w, S.bg));  // pathtracer.cpp:26-28
+                if (records) A.rec_x[slot] = S.bg;
             } else {
                 has_hit = true;
                 const uint32_t id = h.x;
@@ -373,18 +429,18 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                 p2 = make_float3(p.x + 0.0001f * nn.x, p.y + 0.0001f * nn.y, p.z + 0.0001f * nn.z);
                 rho = A.albedo[id];
                 const float4 wr = mul4(w, rho);
-                if (A.nlights) {  // pathtracer.cpp:44-58
-                    float3 l = make_float3(A.light_pos.x - p.x, A.light_pos.y - p.y, A.light_pos.z - p.z);
+                if (S.nlights) {  // pathtracer.cpp:44-58
+                    float3 l = make_float3(S.light_pos.x - p.x, S.light_pos.y - p.y, S.light_pos.z - p.z);
                     const float il = 1.0f / sqrtf(l.x * l.x + l.y * l.y + l.z * l.z);
                     ldir = make_float3(il * l.x, il * l.y, il * l.z);
-                    const float3 l2 = make_float3(A.light_pos.x - p2.x, A.light_pos.y - p2.y, A.light_pos.z - p2.z);
+                    const float3 l2 = make_float3(S.light_pos.x - p2.x, S.light_pos.y - p2.y, S.light_pos.z - p2.z);
                     ldist = sqrtf(l2.x * l2.x + l2.y * l2.y + l2.z * l2.z);
                     const float c = fmaxf(0.f, ldir.x * nn.x + ldir.y * nn.y + ldir.z * nn.z);
-                    lcontrib = scale4(inv_pi, mul4(wr, scale4(c, A.light_col)));
-                    if (records) llocal = scale4(inv_pi, mul4(rho, scale4(c, A.light_col)));
+                    lcontrib = scale4(inv_pi, mul4(wr, scale4(c, S.light_col)));
+                    if (records) llocal = scale4(inv_pi, mul4(rho, scale4(c, S.light_col)));
                     has_point = true;
                 }
-                if (area) {
+                if (area_flag && S.nemit > 0) {
                     const float4 le = A.emission[id];
                     const float4 g = A.trirec[4 * (size_t)id + 1];  // geometric normal
                     if (level == 0 && (le.x > 0 || le.y > 0 || le.z > 0) &&
@@ -392,14 +448,15 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                         accumulate(A.accum, pixel, mul4(w, le));
                     // one emitter by area, uniform point on it
                     const float u0 = rng_uniform(rng), u1 = rng_uniform(rng), u2 = rng_uniform(rng);
-                    const float x = u0 * A.emit_area;
-                    uint32_t lo = 0, hi = A.nemit - 1;
+                    const float x = u0 * S.emit_area;
+                    const float* cdf = A.emit_cdf + S.emit_begin;
+                    uint32_t lo = 0, hi = S.nemit - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
-                        if (A.emit_cdf[mid] <= x) lo = mid + 1;
+                        if (cdf[mid] <= x) lo = mid + 1;
                         else hi = mid;
                     }
-                    const uint32_t k = A.emit_ids[lo];
+                    const uint32_t k = A.emit_ids[S.emit_begin + lo];
                     const float4 e0 = A.trirec[4 * (size_t)k], en = A.trirec[4 * (size_t)k + 1],
                                  eu = A.trirec[4 * (size_t)k + 2], ev = A.trirec[4 * (size_t)k + 3];
                     const float su = sqrtf(u1);
@@ -416,7 +473,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                         const float3 q2 = make_float3(q.x - p2.x, q.y - p2.y, q.z - p2.z);
                         adist = sqrtf(q2.x * q2.x + q2.y * q2.y + q2.z * q2.z) * (1.0f - 1e-4f);
                         const float4 lek = A.emission[k];
-                        const float gterm = cosx * cosl / d2 * A.emit_area;
+                        const float gterm = cosx * cosl / d2 * S.emit_area;
                         acontrib = scale4(inv_pi, mul4(wr, scale4(gterm, lek)));
                         if (records) alocal = scale4(inv_pi, mul4(rho, scale4(gterm, lek)));
                         has_area = true;
@@ -544,9 +601,9 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
 }
 
 // Clears the training records of a chunk: valid flags and found radiance.
-__global__ void k_rec_clear(RenderArgs A, uint32_t slots) {
+__global__ void k_rec_clear(RenderArgs A) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= slots) return;
+    if (k >= A.cs->count * A.rstride) return;
     A.rec_n[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     A.rec_x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -555,8 +612,9 @@ __global__ void k_rec_clear(RenderArgs A, uint32_t slots) {
 // and appends (p, n, w, lum(L_k) cos / pdf) of every bounce that brought
 // light back to the training set (kept with probability A.accept).  Sample
 // layout: (p, c), (n, cos / pdf), (w, path << 6 | level as bits).
-__global__ __launch_bounds__(kRBlock) void k_collect(RenderArgs A, uint32_t npaths, int levels_with_records,
-                                                     uint64_t salt) {
+__global__ __launch_bounds__(kRBlock) void k_collect(RenderArgs A, int levels_with_records) {
+    const uint32_t npaths = A.cs->count;
+    const uint64_t salt = A.seed_pass ^ (A.cs->first * 0x9E3779B97F4A7C15ull);
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const bool act = pid < npaths;
@@ -626,13 +684,16 @@ struct RenderBuffers {
     Queues q{};
     size_t bytes = 0;
     char* base = nullptr;
+    ChunkState* cs = nullptr;
+    SceneParams* sp = nullptr;
+    uint32_t sp_cap = 0;
 };
 
 std::mutex g_render_mutex;
 
 // Carves the queue buffers out of one allocation; with base == nullptr it
 // only measures (every piece is rounded up to 256 bytes).
-size_t carve(Queues& q, char* base, uint32_t cap) {
+size_t carve(Queues& q, ChunkState** cs, char* base, uint32_t cap) {
     size_t off = 0;
     auto take = [&](size_t b) {
         char* r = base ? base + off : nullptr;
@@ -653,20 +714,30 @@ size_t carve(Queues& q, char* base, uint32_t cap) {
     q.sx = (float4*)take((size_t)q.scap * 16);
     q.cnt = (uint32_t*)take((4 * kMaxLevels + 1) * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
+    *cs = (ChunkState*)take(sizeof(ChunkState));
     q.cap = cap;
     return off;
 }
 
-int alloc_queues(RenderBuffers& rb, uint32_t cap) {
+int alloc_queues(RenderBuffers& rb, uint32_t cap, uint32_t nscenes) {
+    if (!rb.sp || rb.sp_cap < nscenes) {
+        if (rb.sp) (void)hipFree(rb.sp);
+        rb.sp = nullptr;
+        rb.sp_cap = 0;
+        const uint32_t want = std::max(nscenes, 64u);
+        TN_HIP(hipMalloc(&rb.sp, (size_t)want * sizeof(SceneParams)));
+        rb.sp_cap = want;
+    }
     Queues probe{};
-    const size_t need = carve(probe, nullptr, cap);
+    ChunkState* pcs = nullptr;
+    const size_t need = carve(probe, &pcs, nullptr, cap);
     if (rb.base && rb.bytes >= need && rb.q.cap >= cap) return TN_OK;
     if (rb.base) (void)hipFree(rb.base);
     rb.base = nullptr;
     rb.bytes = 0;
     TN_HIP(hipMalloc(&rb.base, need));
     rb.bytes = need;
-    const size_t used = carve(rb.q, rb.base, cap);
+    const size_t used = carve(rb.q, &rb.cs, rb.base, cap);
     return used <= need ? TN_OK : set_error(TN_ERR_INVALID, "queue layout exceeds its allocation");
 }
 
@@ -685,76 +756,144 @@ int alloc_records(RecordBuffers& rb, size_t slots) {
     return TN_OK;
 }
 
+__global__ void k_chunk_setup(ChunkState* cs, unsigned long long total, uint32_t chunk) {
+    if (threadIdx.x == 0) {
+        cs->next = 0;
+        cs->total = total;
+        cs->first = 0;
+        cs->chunk = chunk;
+        cs->count = 0;
+    }
+}
+
+// The geometry a render traverses and shades: one scene, or the packed
+// forest of a batch.
+struct RenderGeom {
+    SceneView sv;
+    const float* normals;
+    const float4* albedo;
+    const float4* emission;
+    const float4* trirec;
+    const uint32_t* emit_ids;
+    const float* emit_cdf;
+    int device;
+    int cus;
+};
+
+RenderGeom geom_of(const tn_scene* sc) {
+    RenderGeom g;
+    g.sv = sc->view();
+    g.normals = sc->d_normals;
+    g.albedo = sc->d_albedo;
+    g.emission = sc->d_emission;
+    g.trirec = sc->d_tris;
+    g.emit_ids = sc->d_emit_ids;
+    g.emit_cdf = sc->d_emit_cdf;
+    g.device = sc->device;
+    g.cus = sc->cus;
+    return g;
+}
+
+SceneParams scene_params(const tn_camera* cam, const tn_light* lights, uint32_t nl, uint32_t width,
+                         uint32_t height, uint32_t spp, const float bg[4], uint32_t emit_begin, uint32_t nemit,
+                         float emit_area) {
+    SceneParams S{};
+    if (cam) {
+        S.cam_pos = make_float4(cam->pos[0], cam->pos[1], cam->pos[2], 0);
+        std::memcpy(S.cam_m, cam->trafo, sizeof S.cam_m);
+        S.cam_dx = cam->delta_x;
+        S.cam_dy = cam->delta_y;
+    }
+    S.width = width;
+    S.height = height;
+    S.spp = spp;
+    S.nlights = nl;
+    if (nl) {
+        S.light_pos = make_float4(lights[0].pos[0], lights[0].pos[1], lights[0].pos[2], 0);
+        S.light_col = make_float4(lights[0].color[0], lights[0].color[1], lights[0].color[2], lights[0].color[3]);
+    }
+    S.bg = make_float4(bg[0], bg[1], bg[2], bg[3]);
+    S.emit_begin = emit_begin;
+    S.nemit = nemit;
+    S.emit_area = emit_area;
+    return S;
+}
+
 }  // namespace
 
-// Renders params->width x height x spp samples, adding the per-pixel SUM of
-// sample radiance into accum (device, RGBA).
-int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
-                const tn_render_params* P, float* accum, tn_render_stats* st, hipStream_t stream,
-                const float* ray_o = nullptr, const float* ray_d = nullptr, uint64_t nrays = 0,
-                tn_guide* guide = nullptr, uint32_t train_samples = 0) {
+// Launch shape of the wavefront loop: primary samples per chunk (0: from the
+// queue budget), blocks per CU of the persistent trace / shade / shadow
+// kernels, and whether the per-chunk launch sequence is captured once in a
+// hipGraph and replayed for every chunk.
+struct LaunchShape {
+    uint32_t chunk = 0;
+    uint32_t blocks_per_cu = 8;
+    bool graph = false;
+};
+
+int check_render_params(const tn_render_params* P) {
+    if (P->max_depth < 0 || P->mc_samples < 0) return set_error(TN_ERR_INVALID, "invalid render parameters");
+    if ((uint32_t)P->max_depth + 2 > kMaxLevels) return set_error(TN_ERR_LIMIT, "max_depth too large");
+    return TN_OK;
+}
+
+// Chunk size for a render: the queue of a chunk's deepest level holds
+// chunk * m^max_depth segments; the default keeps it near 2^23.
+int chunk_size(const tn_render_params* P, uint32_t requested, uint32_t& chunk, uint32_t& cap) {
+    const uint32_t m = (uint32_t)P->mc_samples;
+    double fan = 1;
+    for (int k = 0; k < P->max_depth; ++k) fan *= std::max(1u, m);
+    chunk = requested ? requested : P->chunk_paths;
+    if (chunk == 0) chunk = (uint32_t)std::max(1.0, std::min((double)(1u << 23) / fan, 1.0 * (1u << 22)));
+    if (fan * chunk > 2147483647.0) return set_error(TN_ERR_LIMIT, "queue would exceed 2^31 entries");
+    cap = (uint32_t)std::max(1.0, fan * chunk);
+    return TN_OK;
+}
+
+// The wavefront loop over `total` samples of the scenes in `table` (sample
+// and pixel ranges filled in by the caller), adding per-pixel radiance sums
+// into accum.  Caller rays (ray_o/ray_d) replace the camera for tn_trace.
+int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint64_t total,
+                const tn_render_params* P, const LaunchShape& shape, float* accum, tn_render_stats* st,
+                hipStream_t stream, const float* ray_o, const float* ray_d, const float* jitter_host,
+                size_t jitter_len, tn_guide* guide, uint32_t train_samples) {
     const bool rays = ray_o != nullptr;
     const bool guided = guide != nullptr;
     const bool training = guided && train_samples > 0;
+    int rc = check_render_params(P);
+    if (rc) return rc;
     if ((P->flags & TN_RENDER_GUIDED) && !guided)
         return set_error(TN_ERR_INVALID, "TN_RENDER_GUIDED needs a guide (tn_render_guided)");
     if (training && P->mc_samples != 1)
         return set_error(TN_ERR_INVALID, "training records need mc_samples == 1 (one path per sample)");
-    if (guided && guide->device != sc->device)
+    if (guided && guide->device != G.device)
         return set_error(TN_ERR_INVALID, "guide and scene live on different devices");
-    if ((!rays && (P->width == 0 || P->height == 0 || P->pixel_samples < 1)) || P->max_depth < 0 ||
-        P->mc_samples < 0)
-        return set_error(TN_ERR_INVALID, "invalid render parameters");
-    if (nl > 1) return set_error(TN_ERR_LIMIT, "at most one point light (main.cpp:123)");
-    if ((uint32_t)P->max_depth + 2 > kMaxLevels) return set_error(TN_ERR_LIMIT, "max_depth too large");
-    if (!rays && (size_t)P->width * P->height >= (1ull << 32))
-        return set_error(TN_ERR_LIMIT, "image too large");
-    if (rays && nrays >= (1ull << 32)) return set_error(TN_ERR_LIMIT, "too many rays");
-    const uint32_t m = (uint32_t)P->mc_samples;
-    // queue capacity: a chunk's deepest level holds chunk * m^max_depth segments
-    double fan = 1;
-    for (int k = 0; k < P->max_depth; ++k) fan *= std::max(1u, m);
-    const double cap_target = 1u << 23;
-    uint32_t chunk = P->chunk_paths;
-    if (chunk == 0) chunk = (uint32_t)std::max(1.0, std::min(cap_target / fan, 1.0 * (1u << 22)));
-    if (fan * chunk > 2147483647.0) return set_error(TN_ERR_LIMIT, "queue would exceed 2^31 entries");
-    const uint32_t cap = (uint32_t)std::max(1.0, fan * chunk);
+    uint32_t chunk = 0, cap = 0;
+    if ((rc = chunk_size(P, shape.chunk, chunk, cap))) return rc;
+    if (st) *st = tn_render_stats{};
+    if (total == 0) return TN_OK;
 
     std::lock_guard<std::mutex> lock(g_render_mutex);
     static RenderBuffers rb;
-    int rc = alloc_queues(rb, cap);
-    if (rc) return rc;
+    if ((rc = alloc_queues(rb, cap, (uint32_t)table.size()))) return rc;
+    TN_HIP(hipMemcpyAsync(rb.sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
+                          stream));
 
     RenderArgs A{};
-    A.sv = sc->view();
-    A.normals = sc->d_normals;
-    A.albedo = sc->d_albedo;
-    A.emission = sc->d_emission;
-    A.trirec = sc->d_tris;
-    A.emit_ids = sc->d_emit_ids;
-    A.emit_cdf = sc->d_emit_cdf;
-    A.nemit = sc->nemit;
-    A.emit_area = sc->emit_area;
-    if (cam) {
-        A.cam_pos = make_float4(cam->pos[0], cam->pos[1], cam->pos[2], 0);
-        std::memcpy(A.cam_m, cam->trafo, sizeof A.cam_m);
-        A.cam_dx = cam->delta_x;
-        A.cam_dy = cam->delta_y;
-    }
-    A.width = P->width;
-    A.height = P->height;
-    A.spp = rays ? 1u : (uint32_t)P->pixel_samples;
+    A.sv = G.sv;
+    A.normals = G.normals;
+    A.albedo = G.albedo;
+    A.emission = G.emission;
+    A.trirec = G.trirec;
+    A.emit_ids = G.emit_ids;
+    A.emit_cdf = G.emit_cdf;
+    A.sp = rb.sp;
+    A.nscenes = (uint32_t)table.size();
+    A.cs = rb.cs;
     A.max_depth = P->max_depth;
     A.mc = P->mc_samples;
     A.flags = P->flags;
     A.seed = P->seed;
-    A.bg = make_float4(P->bg[0], P->bg[1], P->bg[2], P->bg[3]);
-    A.nlights = nl;
-    if (nl) {
-        A.light_pos = make_float4(lights[0].pos[0], lights[0].pos[1], lights[0].pos[2], 0);
-        A.light_col = make_float4(lights[0].color[0], lights[0].color[1], lights[0].color[2],
-                                  lights[0].color[3]);
-    }
     A.accum = (float4*)accum;
     A.seed_pass = P->seed + (uint64_t)P->sample_offset * 0xD1B54A32D192ED03ull;
     if (guided) {
@@ -764,8 +903,7 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     static RecordBuffers recs;
     if (training) {
         A.rstride = (uint32_t)P->max_depth + 1;
-        rc = alloc_records(recs, (size_t)chunk * A.rstride);
-        if (rc) return rc;
+        if ((rc = alloc_records(recs, (size_t)chunk * A.rstride))) return rc;
         const size_t ns = recs.slots;
         A.rec_p = recs.base;
         A.rec_n = recs.base + ns;
@@ -774,73 +912,81 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
         A.rec_x = recs.base + 4 * ns;
         // every bounce of every path is a candidate; keep about kTrainSet of them
         constexpr uint32_t kTrainSet = 1u << 21;
-        rc = guide_reserve_set(guide, kTrainSet, stream);
-        if (rc) return rc;
-        const uint64_t total_paths = rays ? nrays : (uint64_t)P->width * P->height * (uint32_t)P->pixel_samples;
-        const double candidates = (double)total_paths * std::max(1, P->max_depth);
+        if ((rc = guide_reserve_set(guide, kTrainSet, stream))) return rc;
+        const double candidates = (double)total * std::max(1, P->max_depth);
         A.accept = (float)std::min(1.0, 0.95 * kTrainSet / std::max(1.0, candidates));
         A.ts = guide->ts;
         A.ts_count = guide->ts_count;
         A.ts_cap = guide->ts_cap;
     }
     float* d_jit = nullptr;
-    if (!rays && (P->flags & TN_RENDER_REF_JITTER)) {
-        std::vector<float> jit = reference_jitter(P->width, A.spp);
-        TN_HIP(hipMalloc(&d_jit, jit.size() * sizeof(float)));
-        TN_HIP(hipMemcpyAsync(d_jit, jit.data(), jit.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    if (jitter_host) {
+        TN_HIP(hipMalloc(&d_jit, jitter_len * sizeof(float)));
+        TN_HIP(hipMemcpyAsync(d_jit, jitter_host, jitter_len * sizeof(float), hipMemcpyHostToDevice, stream));
         A.jitter = d_jit;
     }
     k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals);
+    k_chunk_setup<<<1, 64, 0, stream>>>(rb.cs, total, chunk);
     const bool exh = P->flags & TN_RENDER_EXHAUSTIVE;
-    const uint64_t total = rays ? nrays : (uint64_t)P->width * P->height * A.spp;
     const int levels = P->max_depth + 1;
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sc->device);
-    const dim3 tgrid(cus * 8), tblock(kRBlock);
-    auto t0 = std::chrono::steady_clock::now();
-    for (uint64_t first = 0; first < total; first += chunk) {
-        const uint32_t cnt0 = (uint32_t)std::min<uint64_t>(chunk, total - first);
-        k_chunk_init<<<1, 4 * kMaxLevels + 64, 0, stream>>>(rb.q.cnt, cnt0);
-        if (training) {
-            const uint32_t slots = cnt0 * A.rstride;
-            k_rec_clear<<<(slots + 255) / 256, 256, 0, stream>>>(A, slots);
-        }
-        if (rays)
-            k_raygen_rays<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, ray_o, ray_d,
-                                                                                first, cnt0);
-        else
-            k_raygen<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(A, rb.q, first, cnt0);
+    const uint32_t bpc = std::max(1u, shape.blocks_per_cu);
+    const dim3 tgrid((uint32_t)G.cus * bpc), tblock(kRBlock);
+    const uint32_t rgrid = (chunk + kRBlock - 1) / kRBlock;
+    const Queues Q = rb.q;
+    auto chunk_sequence = [&](hipStream_t s) {
+        k_chunk_begin<<<1, 4 * kMaxLevels + 64, 0, s>>>(Q.cnt, rb.cs);
+        if (training) k_rec_clear<<<(uint32_t)(((size_t)chunk * A.rstride + 255) / 256), 256, 0, s>>>(A);
+        if (rays) k_raygen_rays<<<rgrid, kRBlock, 0, s>>>(A, Q, ray_o, ray_d);
+        else k_raygen<<<rgrid, kRBlock, 0, s>>>(A, Q);
         int buf = 0;
         for (int lv = 0; lv < levels; ++lv) {
-            if (exh) k_trace<true><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
-            else k_trace<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
-            if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, stream>>>(A, rb.q, lv, buf);
-            else k_shade<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv, buf);
-            if (exh) k_shadow<true><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv);
-            else k_shadow<false><<<tgrid, tblock, 0, stream>>>(A, rb.q, lv);
+            if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            else k_trace<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
+            else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            if (exh) k_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv);
+            else k_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv);
             buf ^= 1;
         }
-        k_chunk_totals<<<1, 64, 0, stream>>>(rb.q.cnt, rb.q.totals, levels);
-        if (training)
-            k_collect<<<(cnt0 + kRBlock - 1) / kRBlock, kRBlock, 0, stream>>>(
-                A, cnt0, P->max_depth, A.seed_pass ^ (first * 0x9E3779B97F4A7C15ull));
+        k_chunk_totals<<<1, 64, 0, s>>>(Q.cnt, Q.totals, levels);
+        if (training) k_collect<<<rgrid, kRBlock, 0, s>>>(A, P->max_depth);
+    };
+    const uint64_t nchunks = (total + chunk - 1) / chunk;
+    auto t0 = std::chrono::steady_clock::now();
+    if (shape.graph && nchunks > 1) {
+        hipStream_t cap_stream = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        TN_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            chunk_sequence(cap_stream);
+            e = hipStreamEndCapture(cap_stream, &graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (uint64_t c = 0; e == hipSuccess && c < nchunks; ++c) e = hipGraphLaunch(exec, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cap_stream);
+        if (e != hipSuccess) {
+            if (d_jit) (void)hipFree(d_jit);
+            return hip_error(e, "captured chunk sequence");
+        }
+    } else {
+        for (uint64_t c = 0; c < nchunks; ++c) chunk_sequence(stream);
     }
     TN_HIP(hipGetLastError());
-    if (training) {
-        rc = guide_train_from_set(guide, train_samples, stream);
-        if (rc) return rc;
-    }
+    if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return rc;
     unsigned long long tot[2] = {0, 0};
     uint32_t overflow = 0;
-    TN_HIP(hipMemcpyAsync(tot, rb.q.totals, 16, hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipMemcpyAsync(&overflow, rb.q.cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipMemcpyAsync(tot, Q.totals, 16, hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipMemcpyAsync(&overflow, Q.cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
     TN_HIP(hipStreamSynchronize(stream));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (d_jit) (void)hipFree(d_jit);
     if (overflow) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
-    uint32_t status = 0;
-    TN_HIP(hipMemcpy(&status, sc->d_status, 4, hipMemcpyDeviceToHost));
     if (st) {
         st->rays = tot[0];
         st->primary_rays = rays ? 0 : total;
@@ -850,6 +996,171 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     return TN_OK;
 }
 
+// Renders params->width x height x spp samples of one scene, adding the
+// per-pixel SUM of sample radiance into accum (device, RGBA).
+int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
+                const tn_render_params* P, float* accum, tn_render_stats* st, hipStream_t stream,
+                const float* ray_o = nullptr, const float* ray_d = nullptr, uint64_t nrays = 0,
+                tn_guide* guide = nullptr, uint32_t train_samples = 0) {
+    const bool rays = ray_o != nullptr;
+    if ((!rays && (P->width == 0 || P->height == 0 || P->pixel_samples < 1)) || P->max_depth < 0 ||
+        P->mc_samples < 0)
+        return set_error(TN_ERR_INVALID, "invalid render parameters");
+    if (nl > 1) return set_error(TN_ERR_LIMIT, "at most one point light (main.cpp:123)");
+    if (!rays && (size_t)P->width * P->height >= (1ull << 32))
+        return set_error(TN_ERR_LIMIT, "image too large");
+    if (rays && nrays >= (1ull << 32)) return set_error(TN_ERR_LIMIT, "too many rays");
+    const uint32_t spp = rays ? 1u : (uint32_t)P->pixel_samples;
+    std::vector<SceneParams> table(1, scene_params(cam, lights, nl, rays ? 1u : P->width, rays ? 1u : P->height,
+                                                   spp, P->bg, 0, sc->nemit, sc->emit_area));
+    const uint64_t total = rays ? nrays : (uint64_t)P->width * P->height * spp;
+    std::vector<float> jit;
+    if (!rays && (P->flags & TN_RENDER_REF_JITTER)) jit = reference_jitter(P->width, spp);
+    LaunchShape shape;
+    shape.graph = (P->flags & TN_RENDER_GRAPH) != 0;
+    return render_core(geom_of(sc), table, total, P, shape, accum, st, stream, ray_o, ray_d,
+                       jit.empty() ? nullptr : jit.data(), jit.size(), guide, train_samples);
+}
+
+// ---------------------------------------------------------------- batches
+namespace {
+
+// Node fixup for the packed forest: right-child indices move by the scene's
+// node base, leaf reference offsets by its reference base.
+void append_tree(const HostTree& t, uint32_t node_base, uint32_t ref_base, uint32_t tri_base,
+                 std::vector<DevNode>& nodes, std::vector<uint32_t>& refs) {
+    for (DevNode n : t.nodes) {
+        if ((n.y & 3u) == 3u) n.x += ref_base;
+        else n.y += node_base << 2;
+        nodes.push_back(n);
+    }
+    for (uint32_t r : t.refs) refs.push_back(r + tri_base);
+}
+
+}  // namespace
+}  // namespace tn
+
+// Scenes packed into one forest: node, reference, triangle and shading arrays
+// concatenated (indices rebased), one root per scene.
+struct tn_batch {
+    int device = 0, cus = 256;
+    uint32_t nscenes = 0;
+    uint2* nodes = nullptr;
+    uint32_t* refs = nullptr;
+    float4* tris = nullptr;
+    float* normals = nullptr;
+    float4* albedo = nullptr;
+    float4* emission = nullptr;
+    uint32_t* emit_ids = nullptr;
+    float* emit_cdf = nullptr;
+    tn::SceneRoot* roots = nullptr;
+    std::vector<uint32_t> emit_begin, nemit;
+    std::vector<float> emit_area;
+    struct Tuned {
+        int32_t depth, mc;
+        uint32_t flags;
+        int total_log2;
+        tn::LaunchShape shape;
+        tn_batch_tuning info;
+    };
+    std::vector<Tuned> tuned;
+    tn_batch_tuning last{};
+
+    tn::RenderGeom geom() const {
+        tn::RenderGeom g;
+        g.sv.nodes = nodes;
+        g.sv.refs = refs;
+        g.sv.tris = tris;
+        for (int k = 0; k < 3; ++k) g.sv.bmin[k] = g.sv.bmax[k] = 0.f;
+        g.sv.roots = roots;
+        g.normals = normals;
+        g.albedo = albedo;
+        g.emission = emission;
+        g.trirec = tris;
+        g.emit_ids = emit_ids;
+        g.emit_cdf = emit_cdf;
+        g.device = device;
+        g.cus = cus;
+        return g;
+    }
+};
+
+namespace tn {
+namespace {
+
+void free_batch(tn_batch* b) {
+    for (void* p : {(void*)b->nodes, (void*)b->refs, (void*)b->tris, (void*)b->normals, (void*)b->albedo,
+                    (void*)b->emission, (void*)b->emit_ids, (void*)b->emit_cdf, (void*)b->roots})
+        if (p) (void)hipFree(p);
+}
+
+int batch_table(const tn_batch* b, const tn_batch_job* jobs, std::vector<SceneParams>& table, uint64_t& total,
+                uint64_t& pixels) {
+    table.resize(b->nscenes);
+    total = 0;
+    pixels = 0;
+    for (uint32_t s = 0; s < b->nscenes; ++s) {
+        const tn_batch_job& J = jobs[s];
+        if (J.width == 0 || J.height == 0 || J.pixel_samples < 1 || J.num_lights > 1)
+            return set_error(TN_ERR_INVALID, "invalid job %u", s);
+        table[s] = scene_params(&J.cam, &J.light, J.num_lights, J.width, J.height, (uint32_t)J.pixel_samples,
+                                J.bg, b->emit_begin[s], b->nemit[s], b->emit_area[s]);
+        table[s].sample_begin = total;
+        table[s].pixel_begin = (uint32_t)pixels;
+        total += (uint64_t)J.width * J.height * (uint32_t)J.pixel_samples;
+        pixels += (uint64_t)J.width * J.height;
+        if (pixels >= (1ull << 32)) return set_error(TN_ERR_LIMIT, "batch images exceed 2^32 pixels");
+    }
+    return TN_OK;
+}
+
+// Times candidate launch shapes on the first samples of the batch (into a
+// scratch accumulator) and returns the fastest.
+int autotune(tn_batch* b, const std::vector<SceneParams>& table, uint64_t total, uint64_t pixels,
+             const tn_render_params* P, hipStream_t stream, LaunchShape& best, tn_batch_tuning& info) {
+    const uint64_t probe = std::min<uint64_t>(total, 1ull << 21);
+    float* scratch = nullptr;
+    TN_HIP(hipMalloc(&scratch, (size_t)pixels * 16));
+    const uint32_t chunks[] = {1u << 18, 1u << 20, 1u << 22};
+    const uint32_t grids[] = {4, 8, 16};
+    double best_rate = -1;
+    info = tn_batch_tuning{};
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = TN_OK;
+    for (uint32_t c : chunks) {
+        uint32_t ch = 0, cap = 0;
+        if (chunk_size(P, c, ch, cap) != TN_OK || cap > (1u << 24)) continue;  // queue budget
+        if (c > 2 * probe && c != chunks[0]) continue;
+        for (uint32_t g : grids) {
+            LaunchShape sh;
+            sh.chunk = c;
+            sh.blocks_per_cu = g;
+            sh.graph = true;
+            tn_render_stats st{};
+            rc = render_core(b->geom(), table, probe, P, sh, scratch, &st, stream, nullptr, nullptr, nullptr, 0,
+                             nullptr, 0);
+            if (rc) break;
+            const double rate = (double)probe / std::max(st.ms, 1e-3);
+            ++info.candidates;
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = sh;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipFree(scratch);
+    if (rc) return rc;
+    if (best_rate < 0) return set_error(TN_ERR_LIMIT, "no launch shape fits the queue budget");
+    info.chunk_paths = best.chunk;
+    info.blocks_per_cu = best.blocks_per_cu;
+    info.graph = best.graph ? 1u : 0u;
+    info.best_rate = best_rate;
+    info.probe_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return TN_OK;
+}
+
+}  // namespace
 }  // namespace tn
 
 using namespace tn;
@@ -916,6 +1227,143 @@ int tn_render(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32
     }
     (void)hipFree(acc);
     return rc;
+}
+
+int tn_batch_create(tn_scene* const* scenes, uint32_t n, tn_batch** out) {
+    if (!scenes || !out || n == 0) return set_error(TN_ERR_INVALID, "NULL argument or empty batch");
+    *out = nullptr;
+    for (uint32_t s = 0; s < n; ++s)
+        if (!scenes[s]) return set_error(TN_ERR_INVALID, "NULL scene %u", s);
+    const int dev = scenes[0]->device;
+    for (uint32_t s = 0; s < n; ++s)
+        if (scenes[s]->device != dev) return set_error(TN_ERR_INVALID, "scenes live on different devices");
+    TN_HIP(hipSetDevice(dev));
+    std::vector<DevNode> nodes;
+    std::vector<uint32_t> refs, eids;
+    std::vector<float> ecdf;
+    std::vector<SceneRoot> roots(n);
+    tn_batch* b = new tn_batch();
+    b->device = dev;
+    b->cus = scenes[0]->cus;
+    b->nscenes = n;
+    uint64_t ntri = 0;
+    std::vector<uint64_t> tri_base(n);
+    for (uint32_t s = 0; s < n; ++s) {
+        const tn_scene* sc = scenes[s];
+        tri_base[s] = ntri;
+        if (ntri + sc->ntris >= TN_INVALID_ID || nodes.size() + sc->tree.nodes.size() >= (1ull << 30) ||
+            refs.size() + sc->tree.refs.size() >= (1ull << 32)) {
+            delete b;
+            return set_error(TN_ERR_LIMIT, "batch exceeds 2^30 triangles or nodes");
+        }
+        roots[s].node = (uint32_t)nodes.size();
+        for (int k = 0; k < 3; ++k) {
+            roots[s].bmin[k] = sc->tree.box[k];
+            roots[s].bmax[k] = sc->tree.box[3 + k];
+        }
+        append_tree(sc->tree, (uint32_t)nodes.size(), (uint32_t)refs.size(), (uint32_t)ntri, nodes, refs);
+        ntri += sc->ntris;
+        b->emit_begin.push_back((uint32_t)eids.size());
+        b->nemit.push_back(sc->nemit);
+        b->emit_area.push_back(sc->emit_area);
+        if (sc->nemit) {
+            std::vector<uint32_t> ids(sc->nemit);
+            std::vector<float> cdf(sc->nemit);
+            hipError_t e = hipMemcpy(ids.data(), sc->d_emit_ids, sc->nemit * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(cdf.data(), sc->d_emit_cdf, sc->nemit * 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                delete b;
+                return hip_error(e, "hipMemcpy");
+            }
+            for (uint32_t& i : ids) i += (uint32_t)tri_base[s];
+            eids.insert(eids.end(), ids.begin(), ids.end());
+            ecdf.insert(ecdf.end(), cdf.begin(), cdf.end());
+        }
+    }
+    auto fail = [&](hipError_t e, const char* what) {
+        free_batch(b);
+        delete b;
+        return hip_error(e, what);
+    };
+    hipError_t e;
+    const size_t nt = std::max<uint64_t>(ntri, 1);
+    if ((e = hipMalloc(&b->nodes, std::max<size_t>(nodes.size(), 1) * 8)) != hipSuccess ||
+        (e = hipMalloc(&b->refs, std::max<size_t>(refs.size(), 1) * 4)) != hipSuccess ||
+        (e = hipMalloc(&b->tris, nt * 64)) != hipSuccess || (e = hipMalloc(&b->normals, nt * 36)) != hipSuccess ||
+        (e = hipMalloc(&b->albedo, nt * 16)) != hipSuccess || (e = hipMalloc(&b->emission, nt * 16)) != hipSuccess ||
+        (e = hipMalloc(&b->emit_ids, std::max<size_t>(eids.size(), 1) * 4)) != hipSuccess ||
+        (e = hipMalloc(&b->emit_cdf, std::max<size_t>(ecdf.size(), 1) * 4)) != hipSuccess ||
+        (e = hipMalloc(&b->roots, n * sizeof(SceneRoot))) != hipSuccess)
+        return fail(e, "hipMalloc");
+    if ((e = hipMemcpy(b->nodes, nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->refs, refs.data(), refs.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->roots, roots.data(), n * sizeof(SceneRoot), hipMemcpyHostToDevice)) != hipSuccess ||
+        (!eids.empty() &&
+         (e = hipMemcpy(b->emit_ids, eids.data(), eids.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (!ecdf.empty() &&
+         (e = hipMemcpy(b->emit_cdf, ecdf.data(), ecdf.size() * 4, hipMemcpyHostToDevice)) != hipSuccess))
+        return fail(e, "hipMemcpy");
+    for (uint32_t s = 0; s < n; ++s) {
+        const tn_scene* sc = scenes[s];
+        const size_t t0 = tri_base[s], k = sc->ntris;
+        if ((e = hipMemcpy(b->tris + 4 * t0, sc->d_tris, k * 64, hipMemcpyDeviceToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->normals + 9 * t0, sc->d_normals, k * 36, hipMemcpyDeviceToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->albedo + t0, sc->d_albedo, k * 16, hipMemcpyDeviceToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->emission + t0, sc->d_emission, k * 16, hipMemcpyDeviceToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy");
+    }
+    *out = b;
+    return TN_OK;
+}
+
+int tn_batch_destroy(tn_batch* b) {
+    if (!b) return TN_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    free_batch(b);
+    delete b;
+    return TN_OK;
+}
+
+int tn_batch_render(tn_batch* b, const tn_batch_job* jobs, const tn_render_params* P, float* accum,
+                    tn_render_stats* st, void* stream) {
+    if (!b || !jobs || !P || !accum) return set_error(TN_ERR_INVALID, "NULL argument");
+    if (P->flags & (TN_RENDER_GUIDED | TN_RENDER_REF_JITTER))
+        return set_error(TN_ERR_INVALID, "batches render without guide or reference jitter");
+    int rc = check_render_params(P);
+    if (rc) return rc;
+    TN_HIP(hipSetDevice(b->device));
+    std::vector<SceneParams> table;
+    uint64_t total = 0, pixels = 0;
+    if ((rc = batch_table(b, jobs, table, total, pixels))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LaunchShape shape;
+    shape.graph = (P->flags & TN_RENDER_GRAPH) != 0;
+    if ((P->flags & TN_RENDER_AUTOTUNE) && P->chunk_paths == 0) {
+        const uint32_t key_flags = P->flags & ~(TN_RENDER_GRAPH | TN_RENDER_AUTOTUNE);
+        int lg = 0;
+        while ((1ull << lg) < total) ++lg;
+        const tn_batch::Tuned* hit = nullptr;
+        for (const auto& t : b->tuned)
+            if (t.depth == P->max_depth && t.mc == P->mc_samples && t.flags == key_flags && t.total_log2 == lg)
+                hit = &t;
+        if (hit) {
+            shape = hit->shape;
+            b->last = hit->info;
+        } else {
+            tn_batch_tuning info{};
+            if ((rc = autotune(b, table, total, pixels, P, s, shape, info))) return rc;
+            b->tuned.push_back({P->max_depth, P->mc_samples, key_flags, lg, shape, info});
+            b->last = info;
+        }
+    }
+    return render_core(b->geom(), table, total, P, shape, accum, st, s, nullptr, nullptr, nullptr, 0, nullptr, 0);
+}
+
+int tn_batch_get_tuning(const tn_batch* b, tn_batch_tuning* out) {
+    if (!b || !out) return set_error(TN_ERR_INVALID, "NULL argument");
+    *out = b->last;
+    return TN_OK;
 }
 
 }  // extern "C"

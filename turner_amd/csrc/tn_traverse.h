@@ -43,12 +43,22 @@ constexpr float kDirEps = 0.00001f;  // lib/types.h:14
 constexpr float kExitMargin = 1.0f + 1.0f / 65536.0f;
 constexpr int kMaxRestarts = 4096;
 
+// Root of one tree in a packed forest (many scenes in one node / reference /
+// triangle allocation): its node index and root box.
+struct SceneRoot {
+    uint32_t node;
+    float bmin[3];
+    float bmax[3];
+    uint32_t pad;
+};
+
 struct SceneView {
     const uint2* nodes;
     const uint32_t* refs;
     const float4* tris;  // 4 per triangle
     float bmin[3];
     float bmax[3];
+    const SceneRoot* roots = nullptr;  // forest: per-scene roots (bmin/bmax unused)
 };
 
 struct TraversalCounters {
@@ -60,7 +70,7 @@ struct TraversalCounters {
 struct Trav {
     float3 o, d, dinv;
     float tenter, texit, root_exit;
-    uint32_t node;
+    uint32_t node, root;
     uint32_t leaf_off, leaf_left;  // triangle references still to test in the current leaf
     int head, count, restarts;
     bool lost;
@@ -160,9 +170,11 @@ __device__ __forceinline__ int leaf_test_mt(const float4* __restrict__ T, uint32
     return 2;
 }
 
-// Fixes the direction and clips the ray to the root box (kdtree.cpp:519-535).
-// Returns false when the root box is missed.
-__device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o, float3 d) {
+// Fixes the direction and clips the ray to the root box (kdtree.cpp:519-535)
+// of its scene (the tree of `scene` in a forest).  Returns false when the root
+// box is missed.
+__device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o, float3 d,
+                                          uint32_t scene = 0) {
     T.o = o;
     T.d = d;
     float3 fd = d;
@@ -171,18 +183,33 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     if (fd.z == 0.f) fd.z = kDirEps;
     T.dinv = make_float3(1 / fd.x, 1 / fd.y, 1 / fd.z);
     const float3 di = T.dinv;
-    const float tx1 = (sc.bmin[0] - o.x) * di.x, tx2 = (sc.bmax[0] - o.x) * di.x;
+    float b0[3], b1[3];
+    if (sc.roots) {
+        const SceneRoot& R = sc.roots[scene];
+        T.root = R.node;
+        for (int k = 0; k < 3; ++k) {
+            b0[k] = R.bmin[k];
+            b1[k] = R.bmax[k];
+        }
+    } else {
+        T.root = 0;
+        for (int k = 0; k < 3; ++k) {
+            b0[k] = sc.bmin[k];
+            b1[k] = sc.bmax[k];
+        }
+    }
+    const float tx1 = (b0[0] - o.x) * di.x, tx2 = (b1[0] - o.x) * di.x;
     float lo = fminf(tx1, tx2), hi = fmaxf(tx1, tx2);
-    const float ty1 = (sc.bmin[1] - o.y) * di.y, ty2 = (sc.bmax[1] - o.y) * di.y;
+    const float ty1 = (b0[1] - o.y) * di.y, ty2 = (b1[1] - o.y) * di.y;
     lo = fmaxf(lo, fminf(ty1, ty2));
     hi = fminf(hi, fmaxf(ty1, ty2));
-    const float tz1 = (sc.bmin[2] - o.z) * di.z, tz2 = (sc.bmax[2] - o.z) * di.z;
+    const float tz1 = (b0[2] - o.z) * di.z, tz2 = (b1[2] - o.z) * di.z;
     lo = fmaxf(lo, fminf(tz1, tz2));
     hi = fminf(hi, fmaxf(tz1, tz2));
     T.tenter = lo;
     T.texit = hi;
     T.root_exit = hi;
-    T.node = 0;
+    T.node = T.root;
     T.leaf_off = 0;
     T.leaf_left = 0;
     T.head = 0;
@@ -251,7 +278,7 @@ __device__ __forceinline__ bool trav_after_leaf(Trav& T, float tmax_any, const u
             cnt.incomplete = true;
             return true;
         }
-        T.node = 0;
+        T.node = T.root;
         T.tenter = texit;
         T.texit = T.root_exit;
         T.lost = false;
@@ -331,7 +358,7 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
             cnt.incomplete = true;
             return true;
         }
-        T.node = 0;
+        T.node = T.root;
         T.tenter = texit;
         T.texit = T.root_exit;
         T.lost = false;
@@ -439,7 +466,7 @@ leaf_done : {
             cnt.incomplete = true;
             return true;
         }
-        T.node = 0;
+        T.node = T.root;
         T.tenter = texit;
         T.texit = T.root_exit;
         T.lost = false;
@@ -498,7 +525,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 // Finished lanes take the next rays of the wave's chunk.
 //   chunk(begin, end): called by all lanes of the wave; returns false when no
 //                      work is left (may update begin/end);
-//   load(k, o, d, tmax), store(k, T, traversed): per-ray I/O.
+//   load(k, o, d, tmax, scene), store(k, T, traversed): per-ray I/O (scene:
+//                      the ray's tree in a forest, 0 otherwise).
 template <int S, bool EXH, bool MT, bool ANY, bool COUNT, class Chunk, class Load, class Store>
 __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int stride,
                                           unsigned long long* wslot, Chunk chunk, Load load,
@@ -517,8 +545,9 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
             next += (uint32_t)__popcll(idle);
             if (ray == kNone && k < end) {
                 float3 o, d;
-                load(k, o, d, tmx);
-                if (trav_init(T, sc, o, d)) ray = k;
+                uint32_t scene = 0;
+                load(k, o, d, tmx, scene);
+                if (trav_init(T, sc, o, d, scene)) ray = k;
                 else store(k, T, false);
             }
         }

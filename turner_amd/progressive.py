@@ -26,18 +26,28 @@ def _conf(width, height, depth, flags, spp, seed):
 
 def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=16,
                    train_passes=16, train_samples=1 << 18, train_spp=1, max_seconds=60.0, seed=1,
-                   max_spp=1 << 16, ref_var=None):
+                   max_spp=1 << 16, ref_var=None, keep_training_passes=True, train_every=0):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
-    reference's per-pixel variance (relmse subtracts it)."""
+    reference's per-pixel variance (relmse subtracts it).
+    keep_training_passes=False leaves the passes rendered while the guide is
+    still training out of the image once a later pass exists (a choice fixed
+    in advance, so the image stays unbiased); their time still counts.
+    train_every=k > 0 keeps training after that phase on every k-th image
+    pass (those passes are rendered before the update and stay in the
+    image)."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    tacc = acc if keep_training_passes else torch.zeros_like(acc)
     spent, spp, rays, hist = 0.0, 0, 0, []
+    spp_img = 0  # samples in acc
     err = float("inf")
     k = 0
     while spent < max_seconds and spp < max_spp:
         training = guide is not None and k < train_passes
+        late = (guide is not None and not training and train_every > 0
+                and (k - train_passes) % train_every == 0)
         n = train_spp if training else spp_per_pass
         conf = _conf(w, h, depth, flags, n, seed)
         torch.cuda.synchronize()
@@ -45,14 +55,17 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
         if guide is None:
             st = render_accumulate(tree, scene["camera"], scene["lights"], conf, acc, sample_offset=spp)
         else:
-            st = render_guided(tree, scene["camera"], scene["lights"], conf, guide, acc,
-                               train_samples=train_samples if training else 0, sample_offset=spp)
+            st = render_guided(tree, scene["camera"], scene["lights"], conf, guide, tacc if training else acc,
+                               train_samples=train_samples if (training or late) else 0, sample_offset=spp)
         torch.cuda.synchronize()
         spent += time.perf_counter() - t0
         spp += n
+        if acc is tacc or not training:
+            spp_img += n
         rays += st["rays"] + st["shadow_rays"]
         k += 1
-        err = relmse(acc / spp, ref, ref_var)
+        img = acc / spp_img if spp_img else tacc / spp
+        err = relmse(img, ref, ref_var)
         hist.append((spent, spp, err))
         if err <= target:
             break
