@@ -51,6 +51,17 @@ typedef struct {
 
 /* Scene creation flags */
 #define TN_SCENE_SERIAL_BUILD 1u /* build the kd-tree on one host thread */
+/* Rendering tree: the reference's SAH build with two quirks removed.  A
+ * triangle that clipping removes from a node entirely keeps, in the
+ * reference, an unbounded box (lib/clipping.h:210-212) and so sinks into
+ * every descendant, filling leaves with triangles that do not touch them; a
+ * split whose one side holds no triangle is dropped (lib/kdtree.cpp:166-171),
+ * so the other subtree also spans the empty space.  With this flag such
+ * triangles are dropped and empty sides become empty leaves.  Closest hits
+ * equal the reference tree's except between exactly equal distances (visit
+ * order differs); the scene has no reference FlatNode array
+ * (tn_scene_get_flat_nodes fails).  The reference tree stays the default. */
+#define TN_SCENE_EMPTY_LEAVES 2u
 
 /*
  * Replaces KDTree::KDTree(Triangles) (lib/kdtree.h:195, lib/kdtree.cpp:474-490):
@@ -84,6 +95,14 @@ int tn_scene_get_triangle_records(const tn_scene* scene, float* out16, size_t ca
 int tn_kdtree_build(const float* verts9, uint32_t num_triangles, uint32_t flags,
                     uint64_t** flat_nodes, size_t* num_nodes, float* box6, uint32_t* height);
 void tn_buffer_free(void* p);
+/* Host-only: the device layout of the tree tn_scene_create would upload (with
+ * flags TN_SCENE_SERIAL_BUILD / TN_SCENE_EMPTY_LEAVES): nodes2 holds two
+ * uint32 per node -- inner (split bits, right << 2 | axis), leaf (offset into
+ * refs, count << 2 | 3), left child = next node -- and refs the leaves'
+ * triangle ids.  Free both with tn_buffer_free. */
+int tn_kdtree_build_device(const float* verts9, uint32_t num_triangles, uint32_t flags,
+                           uint32_t** nodes2, size_t* num_nodes, uint32_t** refs, size_t* num_refs,
+                           float* box6, uint32_t* height);
 
 /* kd-tree cache replacing ./kdtree.cache (main.cpp:144-167).  Unlike the
  * reference's cache, the file is versioned and keyed by a 64-bit hash of the

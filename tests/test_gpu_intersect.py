@@ -240,3 +240,25 @@ def test_full_config_ray_batch(gpu):
     got = tuple(x.cpu().numpy()[sub] for x in a)
     got = (got[0].view(np.uint32), got[1], got[2], got[3])
     assert_bit_exact(got, ref, "full subset")
+
+
+def test_render_tree_gives_the_same_hits(gpu):
+    """TN_SCENE_EMPTY_LEAVES (rendering tree): the closest hit distance of
+    every ray equals the reference tree's bit for bit; ids may differ only
+    between triangles at exactly that distance (shared edges)."""
+    for sc in (scenes.occluded_light_cornell(level=5), scenes.intersect_scene(level=4)):
+        ref = T.KDTree(sc["verts"])
+        ren = T.KDTree(sc["verts"], empty_leaves=True)
+        lo, hi = np.asarray(ref.box()[:3]), np.asarray(ref.box()[3:])
+        rng = np.random.default_rng(3)
+        n = 1 << 18
+        o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+        d = rng.normal(size=(n, 3)).astype(np.float32)
+        a = T.KDTreeIntersection(ref).intersect_batch(o, d)
+        b = T.KDTreeIntersection(ren).intersect_batch(o, d)
+        assert np.array_equal(a[1], b[1]), "hit distances differ"
+        diff = a[0] != b[0]
+        print(f"[render tree] {len(sc['verts'])} tris: ids differ on {diff.sum()} of {n} rays (ties)", flush=True)
+        assert diff.mean() <= 1e-3
+        ra = a[1][diff]
+        assert np.all(np.isfinite(ra)) and np.all(a[0][diff] != T.TN_INVALID_ID)

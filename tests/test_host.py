@@ -106,3 +106,61 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(T.TurnerError) as e:
         T.KDTree(H.cube12())
     assert e.value.code == T._lib.TN_ERR_NODEVICE
+
+
+def _leaf_of_point(nodes, p):
+    """Leaves (node indices) whose cell holds point p: descend by the split
+    planes, both ways for a point on a plane."""
+    out, st = [], [0]
+    while st:
+        i = st.pop()
+        x, y = int(nodes[i, 0]), int(nodes[i, 1])
+        if (y & 3) == 3:
+            out.append(i)
+            continue
+        ax = y & 3
+        split = np.array([x], np.uint32).view(np.float32)[0]
+        if p[ax] <= split:
+            st.append(i + 1)
+        if p[ax] >= split:
+            st.append(y >> 2)
+    return out
+
+
+@pytest.mark.parametrize("name", ["soup", "grid", "sphere_walls"])
+def test_render_tree_covers_every_triangle(name):
+    """The rendering tree (TN_SCENE_EMPTY_LEAVES) drops fully clipped
+    triangles and keeps empty leaves; it must still be conservative: every
+    point of every triangle lies in a leaf that references the triangle, so a
+    ray hitting a triangle visits a leaf that tests it.  Also: no more
+    references than the reference tree, valid indices, one leaf per cell."""
+    rng = np.random.default_rng(5)
+    if name == "soup":
+        verts = H.random_soup(300, rng)
+    elif name == "grid":
+        verts = H.grid_mesh(12)
+    else:
+        v, _ = scenes.icosphere(3, 0.25, 0.05, 1)
+        v = (v.reshape(-1, 3) + np.array([0.5, 0.3, 0.5], np.float32)).reshape(-1, 9)
+        verts = np.concatenate([scenes.concat(*scenes.unit_cornell_walls())[0], v])
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+    nodes, refs, box, h = T.build_device_tree(verts, empty_leaves=True)
+    rnodes, rrefs, rbox, _ = T.build_device_tree(verts)
+    assert np.array_equal(box, rbox)
+    assert len(refs) <= len(rrefs)
+    inner = (nodes[:, 1] & 3) != 3
+    assert np.all((nodes[inner, 1] >> 2) < len(nodes))
+    leaf = ~inner
+    assert np.all(nodes[leaf, 0] + (nodes[leaf, 1] >> 2) <= len(refs))
+    assert np.all(refs < len(verts))
+    tri = verts.reshape(-1, 3, 3).astype(np.float64)
+    for t in range(len(tri)):
+        for _ in range(6):
+            a, b = rng.uniform(size=2)
+            if a + b > 1:
+                a, b = 1 - a, 1 - b
+            p = (tri[t, 0] + a * (tri[t, 1] - tri[t, 0]) + b * (tri[t, 2] - tri[t, 0])).astype(np.float32)
+            cells = _leaf_of_point(nodes, p)
+            assert cells, (t, p)
+            found = any(t in refs[nodes[c, 0]:nodes[c, 0] + (nodes[c, 1] >> 2)] for c in cells)
+            assert found, f"triangle {t} point {p} not referenced by its leaf"

@@ -40,6 +40,27 @@ def build_tree(verts, serial=False):
     return out, box, h.value
 
 
+def build_device_tree(verts, serial=False, empty_leaves=False):
+    """Host-only: the device layout tn_scene_create would upload -- nodes
+    (M, 2) uint32 (inner: split bits, right << 2 | axis; leaf: offset,
+    count << 2 | 3), refs (K,) uint32, box (6,), height."""
+    v = _f32(verts, 9)
+    pn = C.POINTER(C.c_uint32)()
+    pr = C.POINTER(C.c_uint32)()
+    nn, nr, h = C.c_size_t(), C.c_size_t(), C.c_uint32()
+    box = np.zeros(6, np.float32)
+    flags = (_lib.TN_SCENE_SERIAL_BUILD if serial else 0) | (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0)
+    check(lib().tn_kdtree_build_device(ptr(v), len(v), flags, C.byref(pn), C.byref(nn), C.byref(pr),
+                                       C.byref(nr), ptr(box), C.byref(h)))
+    try:
+        nodes = np.ctypeslib.as_array(pn, shape=(nn.value * 2,)).copy().reshape(-1, 2)
+        refs = np.ctypeslib.as_array(pr, shape=(max(nr.value, 1),)).copy()[:nr.value]
+    finally:
+        lib().tn_buffer_free(pn)
+        lib().tn_buffer_free(pr)
+    return nodes, refs, box, h.value
+
+
 def tree_cache_save(path, verts, nodes, box):
     v = _f32(verts, 9)
     nodes = np.ascontiguousarray(nodes, np.uint64)
@@ -81,10 +102,12 @@ class KDTree:
     verts (N, 9) float32; normals (N, 9) or None (face normals); albedo (N, 4)
     or None (0.8 grey); emission (N, 4) or None.  `flat_nodes`/`box` upload a
     tree built earlier (e.g. from the tree cache) instead of building one.
+    empty_leaves=True builds the rendering variant of the tree that keeps
+    empty-space splits (TN_SCENE_EMPTY_LEAVES; no reference FlatNode array).
     """
 
     def __init__(self, verts, normals=None, albedo=None, emission=None, serial=False,
-                 flat_nodes=None, box=None):
+                 flat_nodes=None, box=None, empty_leaves=False):
         self._v = _f32(verts, 9)
         self._n = _f32(normals, 9)
         self._a = _f32(albedo, 4)
@@ -98,7 +121,8 @@ class KDTree:
                                                   ptr(self._e), n, ptr(fl), len(fl), ptr(bx),
                                                   C.byref(h)))
         else:
-            flags = _lib.TN_SCENE_SERIAL_BUILD if serial else 0
+            flags = (_lib.TN_SCENE_SERIAL_BUILD if serial else 0) | \
+                (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0)
             check(lib().tn_scene_create(ptr(self._v), ptr(self._n), ptr(self._a), ptr(self._e),
                                         n, flags, C.byref(h)))
         self.handle = h

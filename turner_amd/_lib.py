@@ -22,6 +22,7 @@ TN_GUIDE_PARAMS = 15976
 TN_GUIDE_OUTPUTS = 40
 
 TN_SCENE_SERIAL_BUILD = 1
+TN_SCENE_EMPTY_LEAVES = 2
 TN_ISECT_EXHAUSTIVE = 1
 TN_ISECT_MT = 2
 TN_ISECT_COUNT = 4
@@ -110,6 +111,9 @@ SIGNATURES = {
     "tn_kdtree_build": (C.c_int, [_P, _U32, _U32, C.POINTER(C.POINTER(C.c_uint64)),
                                   C.POINTER(C.c_size_t), _P, C.POINTER(C.c_uint32)]),
     "tn_buffer_free": (None, [_P]),
+    "tn_kdtree_build_device": (C.c_int, [_P, _U32, _U32, C.POINTER(C.POINTER(C.c_uint32)),
+                                         C.POINTER(C.c_size_t), C.POINTER(C.POINTER(C.c_uint32)),
+                                         C.POINTER(C.c_size_t), _P, C.POINTER(C.c_uint32)]),
     "tn_tree_cache_save": (C.c_int, [C.c_char_p, _P, _U32, _P, C.c_size_t, _P]),
     "tn_tree_cache_load": (C.c_int, [C.c_char_p, _P, _U32, C.POINTER(C.POINTER(C.c_uint64)),
                                      C.POINTER(C.c_size_t), _P]),

@@ -242,9 +242,11 @@ int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const
                  const float* emission4) {
     const uint32_t n = sc->ntris;
     std::string err;
-    if (!flat_to_device(sc->tree.flat, n, sc->tree.nodes, sc->tree.refs, err))
-        return set_error(TN_ERR_INVALID, "invalid kd-tree: %s", err.c_str());
-    sc->tree.height = tree_height(sc->tree.flat);
+    if (!sc->tree.flat.empty()) {  // else the device layout was built directly
+        if (!flat_to_device(sc->tree.flat, n, sc->tree.nodes, sc->tree.refs, err))
+            return set_error(TN_ERR_INVALID, "invalid kd-tree: %s", err.c_str());
+        sc->tree.height = tree_height(sc->tree.flat);
+    }
     int rc;
     if ((rc = dev_upload(&sc->d_nodes, reinterpret_cast<const uint2*>(sc->tree.nodes.data()),
                          sc->tree.nodes.size())))
@@ -424,7 +426,11 @@ int tn_scene_create(const float* verts9, const float* normals9, const float* alb
     try {
         if ((rc = prepare_triangles(sc.get(), verts9, n))) return rc;
         auto t0 = std::chrono::steady_clock::now();
-        build_kdtree(verts9, n, !(flags & TN_SCENE_SERIAL_BUILD), sc->tree.flat, sc->tree.box);
+        if (flags & TN_SCENE_EMPTY_LEAVES)
+            build_device_tree(verts9, n, !(flags & TN_SCENE_SERIAL_BUILD), sc->tree.nodes, sc->tree.refs,
+                              sc->tree.box, sc->tree.height);
+        else
+            build_kdtree(verts9, n, !(flags & TN_SCENE_SERIAL_BUILD), sc->tree.flat, sc->tree.box);
         sc->tree.build_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if ((rc = finish_scene(sc.get(), verts9, normals9, albedo4, emission4))) {
@@ -479,7 +485,7 @@ int tn_scene_destroy(tn_scene* sc) {
 int tn_scene_get_info(const tn_scene* sc, tn_scene_info* info) {
     if (!sc || !info) return set_error(TN_ERR_INVALID, "NULL argument");
     info->num_triangles = sc->ntris;
-    info->num_nodes = (uint32_t)sc->tree.flat.size();
+    info->num_nodes = (uint32_t)(sc->tree.flat.empty() ? sc->tree.nodes.size() : sc->tree.flat.size());
     info->height = sc->tree.height;
     info->device_nodes = (uint32_t)sc->tree.nodes.size();
     info->device_refs = (uint32_t)sc->tree.refs.size();
@@ -491,6 +497,8 @@ int tn_scene_get_info(const tn_scene* sc, tn_scene_info* info) {
 
 int tn_scene_get_flat_nodes(const tn_scene* sc, uint64_t* out, size_t cap) {
     if (!sc || !out) return set_error(TN_ERR_INVALID, "NULL argument");
+    if (sc->tree.flat.empty())
+        return set_error(TN_ERR_INVALID, "scene built with TN_SCENE_EMPTY_LEAVES has no reference FlatNode array");
     if (cap < sc->tree.flat.size()) return set_error(TN_ERR_LIMIT, "capacity too small");
     std::memcpy(out, sc->tree.flat.data(), sc->tree.flat.size() * sizeof(uint64_t));
     return TN_OK;

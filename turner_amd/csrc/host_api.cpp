@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -51,6 +52,47 @@ int tn_kdtree_build(const float* verts9, uint32_t n, uint32_t flags, uint64_t** 
         *nodes = out;
         *num_nodes = flat.size();
         if (height) *height = tn::tree_height(flat);
+    } catch (const std::bad_alloc&) {
+        return tn::set_error(TN_ERR_OOM, "host allocation failed");
+    }
+    return TN_OK;
+}
+
+int tn_kdtree_build_device(const float* verts9, uint32_t n, uint32_t flags, uint32_t** nodes2,
+                           size_t* num_nodes, uint32_t** refs, size_t* num_refs, float* box6,
+                           uint32_t* height) {
+    if (!verts9 || n == 0 || !nodes2 || !num_nodes || !refs || !num_refs || !box6)
+        return tn::set_error(TN_ERR_INVALID, "NULL argument or no triangles");
+    if (n > TN_MAX_TRIANGLES) return tn::set_error(TN_ERR_LIMIT, "too many triangles");
+    try {
+        std::vector<tn::DevNode> nodes;
+        std::vector<uint32_t> r;
+        uint32_t h = 0;
+        const bool par = !(flags & TN_SCENE_SERIAL_BUILD);
+        if (flags & TN_SCENE_EMPTY_LEAVES) {
+            tn::build_device_tree(verts9, n, par, nodes, r, box6, h);
+        } else {
+            std::vector<uint64_t> flat;
+            tn::build_kdtree(verts9, n, par, flat, box6);
+            std::string err;
+            if (!tn::flat_to_device(flat, n, nodes, r, err))
+                return tn::set_error(TN_ERR_INVALID, "invalid kd-tree: %s", err.c_str());
+            h = tn::tree_height(flat);
+        }
+        uint32_t* on = (uint32_t*)std::malloc(std::max<size_t>(nodes.size(), 1) * 8);
+        uint32_t* orf = (uint32_t*)std::malloc(std::max<size_t>(r.size(), 1) * 4);
+        if (!on || !orf) {
+            std::free(on);
+            std::free(orf);
+            return tn::set_error(TN_ERR_OOM, "host allocation failed");
+        }
+        std::memcpy(on, nodes.data(), nodes.size() * 8);
+        std::memcpy(orf, r.data(), r.size() * 4);
+        *nodes2 = on;
+        *num_nodes = nodes.size();
+        *refs = orf;
+        *num_refs = r.size();
+        if (height) *height = h;
     } catch (const std::bad_alloc&) {
         return tn::set_error(TN_ERR_OOM, "host allocation failed");
     }

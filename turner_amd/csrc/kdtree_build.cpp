@@ -108,7 +108,11 @@ inline int clip_plane(const V3* in, int nin, V3* out, const V3& n, float d) {
     return nout;
 }
 
-inline Box clip_triangle(const float* t, const Box& box) {
+// clipped (optional): set to true when fewer than two polygon points remain
+// -- the triangle does not reach into the box -- and the reference's default
+// box (lib/clipping.h:210-212) is returned.
+inline Box clip_triangle(const float* t, const Box& box, bool* clipped = nullptr) {
+    if (clipped) *clipped = false;
     V3 buf0[40], buf1[40];
     V3* pts = buf0;
     V3* nxt = buf1;
@@ -123,7 +127,10 @@ inline Box clip_triangle(const float* t, const Box& box) {
             float d = s == 0 ? at(box.lo, ax) : -at(box.hi, ax);
             np = clip_plane(pts, np, nxt, n, d);
             std::swap(pts, nxt);
-            if (np < 2) return box_default();
+            if (np < 2) {
+                if (clipped) *clipped = true;
+                return box_default();
+            }
         }
     }
     V3 lo = mk(FLT_MAX, FLT_MAX, FLT_MAX);
@@ -162,13 +169,14 @@ inline float cost_(float lr, float rr, size_t nl, size_t nr) {
 
 class Builder {
 public:
-    Builder(const float* verts, bool parallel) : v_(verts), parallel_(parallel) {
+    Builder(const float* verts, bool parallel, bool keep_empty = false)
+        : v_(verts), parallel_(parallel), keep_empty_(keep_empty) {
         unsigned hc = std::thread::hardware_concurrency();
         budget_.store(parallel ? (int)(hc ? hc : 4) - 1 : 0);
     }
 
     std::unique_ptr<Node> build(std::vector<uint32_t> tris, const Box& box) {
-        if (tris.empty()) return nullptr;
+        if (tris.empty()) return keep_empty_ ? leaf({}) : nullptr;
         if (tris.size() <= 3 || area(box) == 0) return leaf(std::move(tris));
         std::vector<uint32_t> L, R;
         int ax;
@@ -225,8 +233,11 @@ private:
         for (auto& e : ev) e.reserve(2 * tris.size());
         size_t num = 0;
         for (uint32_t id : tris) {
-            Box cb = clip_triangle(v_ + 9 * (size_t)id, box);
-            if (empty(cb)) continue;
+            bool gone = false;
+            Box cb = clip_triangle(v_ + 9 * (size_t)id, box, keep_empty_ ? &gone : nullptr);
+            // the reference keeps a fully clipped triangle with an unbounded box,
+            // so it follows every later split; the rendering variant drops it
+            if (gone || empty(cb)) continue;
             ++num;
             for (int a = 0; a < 3; ++a) {
                 if (planar(cb, a)) {
@@ -303,6 +314,7 @@ private:
 
     const float* v_;
     bool parallel_;
+    bool keep_empty_;  // empty children become empty leaves instead of being dropped
     std::atomic<int> budget_{0};
 };
 
@@ -389,6 +401,65 @@ void build_kdtree(const float* verts9, uint32_t n, bool parallel, std::vector<ui
         kill.pop_back();
         if (n->left) kill.push_back(std::move(n->left));
         if (n->right) kill.push_back(std::move(n->right));
+    }
+    box6[0] = box.lo.x; box6[1] = box.lo.y; box6[2] = box.lo.z;
+    box6[3] = box.hi.x; box6[4] = box.hi.y; box6[5] = box.hi.z;
+}
+
+// The same SAH build with two changes for rendering: a triangle that clipping
+// removes from a node entirely is dropped (the reference gives it an
+// unbounded box, lib/clipping.h:210-212, so it sinks into every descendant
+// and fills leaves with triangles that do not touch them), and a split whose
+// one side is empty keeps that side as an empty leaf (the reference drops it,
+// lib/kdtree.cpp:166-171).  Flattened straight into the device layout.
+void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vector<DevNode>& nodes,
+                       std::vector<uint32_t>& refs, float box6[6], uint32_t& height) {
+    std::vector<uint32_t> ids(n, 0);
+    Box box = tri_box(verts9);
+    for (uint32_t i = 1; i < n; ++i) {
+        Box b = tri_box(verts9 + 9 * (size_t)i);
+        box = box_of(mk(std::min(box.lo.x, b.lo.x), std::min(box.lo.y, b.lo.y),
+                        std::min(box.lo.z, b.lo.z)),
+                     mk(std::max(box.hi.x, b.hi.x), std::max(box.hi.y, b.hi.y),
+                        std::max(box.hi.z, b.hi.z)));
+        ids[i] = i;
+    }
+    Builder B(verts9, parallel, true);
+    std::unique_ptr<Node> root = B.build(std::move(ids), box);
+    nodes.clear();
+    refs.clear();
+    height = 0;
+    // depth-first: left child = next node, right child patched when reached
+    struct Item {
+        const Node* n;
+        uint32_t parent;  // node whose right index this one is, or ~0
+        uint32_t level;
+    };
+    std::vector<Item> st{{root.get(), 0xFFFFFFFFu, 0}};
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        const uint32_t me = (uint32_t)nodes.size();
+        if (it.parent != 0xFFFFFFFFu) nodes[it.parent].y |= me << 2;
+        height = std::max(height, it.level);
+        if (!it.n->leaf) {
+            uint32_t split;
+            std::memcpy(&split, &it.n->split, 4);
+            nodes.push_back(DevNode{split, (uint32_t)it.n->axis});
+            st.push_back({it.n->right.get(), me, it.level + 1});
+            st.push_back({it.n->left.get(), 0xFFFFFFFFu, it.level + 1});
+        } else {
+            nodes.push_back(DevNode{(uint32_t)refs.size(), (uint32_t)it.n->ids.size() << 2 | 3u});
+            refs.insert(refs.end(), it.n->ids.begin(), it.n->ids.end());
+        }
+    }
+    std::vector<std::unique_ptr<Node>> kill;
+    kill.push_back(std::move(root));
+    while (!kill.empty()) {
+        auto k = std::move(kill.back());
+        kill.pop_back();
+        if (k->left) kill.push_back(std::move(k->left));
+        if (k->right) kill.push_back(std::move(k->right));
     }
     box6[0] = box.lo.x; box6[1] = box.lo.y; box6[2] = box.lo.z;
     box6[3] = box.hi.x; box6[4] = box.hi.y; box6[5] = box.hi.z;

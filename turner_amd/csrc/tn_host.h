@@ -39,4 +39,9 @@ bool flat_to_device(const std::vector<uint64_t>& flat, uint32_t num_triangles,
 
 uint32_t tree_height(const std::vector<uint64_t>& flat);
 
+// SAH build that keeps empty children as empty leaves, straight into the
+// device layout (no reference FlatNode array).
+void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vector<DevNode>& nodes,
+                       std::vector<uint32_t>& refs, float box6[6], uint32_t& height);
+
 }  // namespace tn
