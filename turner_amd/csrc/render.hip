@@ -109,6 +109,7 @@ struct RenderArgs {
     const SceneParams* sp;
     uint32_t nscenes;
     ChunkState* cs;
+    uint32_t* status;  // [2]: traversals abandoned as incomplete
     int32_t max_depth, mc;
     uint32_t flags;
     uint64_t seed;
@@ -209,8 +210,9 @@ __global__ void k_chunk_begin(uint32_t* cnt, ChunkState* cs) {
     }
 }
 
-__global__ void k_zero_totals(unsigned long long* totals) {
+__global__ void k_zero_totals(unsigned long long* totals, uint32_t* incomplete) {
     if (threadIdx.x < 2) totals[threadIdx.x] = 0;
+    if (threadIdx.x == 2) *incomplete = 0;
 }
 
 __global__ void k_chunk_totals(const uint32_t* cnt, unsigned long long* totals, int levels) {
@@ -325,6 +327,7 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
                                 __float_as_uint(h ? T.best_s : 0.f), __float_as_uint(h ? T.best_t : 0.f));
         },
         cnt);
+    if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
 }
 
 template <bool EXH>
@@ -357,6 +360,7 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
             }
         },
         cnt);
+    if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
 }
 
 // Orthonormal basis around n (Duff et al. 2017, "Building an Orthonormal
@@ -778,6 +782,7 @@ struct RenderGeom {
     const float* emit_cdf;
     int device;
     int cus;
+    uint32_t* status;  // 4 words; [2] counts incomplete render traversals
 };
 
 RenderGeom geom_of(const tn_scene* sc) {
@@ -791,6 +796,7 @@ RenderGeom geom_of(const tn_scene* sc) {
     g.emit_cdf = sc->d_emit_cdf;
     g.device = sc->device;
     g.cus = sc->cus;
+    g.status = sc->d_status;
     return g;
 }
 
@@ -890,6 +896,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     A.sp = rb.sp;
     A.nscenes = (uint32_t)table.size();
     A.cs = rb.cs;
+    A.status = G.status;
     A.max_depth = P->max_depth;
     A.mc = P->mc_samples;
     A.flags = P->flags;
@@ -925,7 +932,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         TN_HIP(hipMemcpyAsync(d_jit, jitter_host, jitter_len * sizeof(float), hipMemcpyHostToDevice, stream));
         A.jitter = d_jit;
     }
-    k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals);
+    k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals, G.status + 2);
     k_chunk_setup<<<1, 64, 0, stream>>>(rb.cs, total, chunk);
     const bool exh = P->flags & TN_RENDER_EXHAUSTIVE;
     const int levels = P->max_depth + 1;
@@ -979,14 +986,16 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     TN_HIP(hipGetLastError());
     if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return rc;
     unsigned long long tot[2] = {0, 0};
-    uint32_t overflow = 0;
+    uint32_t overflow = 0, incomplete = 0;
     TN_HIP(hipMemcpyAsync(tot, Q.totals, 16, hipMemcpyDeviceToHost, stream));
     TN_HIP(hipMemcpyAsync(&overflow, Q.cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipMemcpyAsync(&incomplete, G.status + 2, 4, hipMemcpyDeviceToHost, stream));
     TN_HIP(hipStreamSynchronize(stream));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (d_jit) (void)hipFree(d_jit);
     if (overflow) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
+    if (incomplete) return set_error(TN_ERR_INCOMPLETE, "%u render traversals hit the visit cap", incomplete);
     if (st) {
         st->rays = tot[0];
         st->primary_rays = rays ? 0 : total;
@@ -1054,6 +1063,7 @@ struct tn_batch {
     uint32_t* emit_ids = nullptr;
     float* emit_cdf = nullptr;
     tn::SceneRoot* roots = nullptr;
+    uint32_t* status = nullptr;
     std::vector<uint32_t> emit_begin, nemit;
     std::vector<float> emit_area;
     struct Tuned {
@@ -1081,6 +1091,7 @@ struct tn_batch {
         g.emit_cdf = emit_cdf;
         g.device = device;
         g.cus = cus;
+        g.status = status;
         return g;
     }
 };
@@ -1090,7 +1101,8 @@ namespace {
 
 void free_batch(tn_batch* b) {
     for (void* p : {(void*)b->nodes, (void*)b->refs, (void*)b->tris, (void*)b->normals, (void*)b->albedo,
-                    (void*)b->emission, (void*)b->emit_ids, (void*)b->emit_cdf, (void*)b->roots})
+                    (void*)b->emission, (void*)b->emit_ids, (void*)b->emit_cdf, (void*)b->roots,
+                    (void*)b->status})
         if (p) (void)hipFree(p);
 }
 
@@ -1293,8 +1305,10 @@ int tn_batch_create(tn_scene* const* scenes, uint32_t n, tn_batch** out) {
         (e = hipMalloc(&b->albedo, nt * 16)) != hipSuccess || (e = hipMalloc(&b->emission, nt * 16)) != hipSuccess ||
         (e = hipMalloc(&b->emit_ids, std::max<size_t>(eids.size(), 1) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->emit_cdf, std::max<size_t>(ecdf.size(), 1) * 4)) != hipSuccess ||
-        (e = hipMalloc(&b->roots, n * sizeof(SceneRoot))) != hipSuccess)
+        (e = hipMalloc(&b->roots, n * sizeof(SceneRoot))) != hipSuccess ||
+        (e = hipMalloc(&b->status, 4 * sizeof(uint32_t))) != hipSuccess)
         return fail(e, "hipMalloc");
+    if ((e = hipMemset(b->status, 0, 4 * sizeof(uint32_t))) != hipSuccess) return fail(e, "hipMemset");
     if ((e = hipMemcpy(b->nodes, nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->refs, refs.data(), refs.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->roots, roots.data(), n * sizeof(SceneRoot), hipMemcpyHostToDevice)) != hipSuccess ||

@@ -42,6 +42,9 @@ constexpr uint32_t kInvalidId = 1u << 30;
 constexpr float kDirEps = 0.00001f;  // lib/types.h:14
 constexpr float kExitMargin = 1.0f + 1.0f / 65536.0f;
 constexpr int kMaxRestarts = 4096;
+// Leaves one ray may visit before its traversal is abandoned as incomplete
+// (far above any real ray; bounds every traversal loop).
+constexpr uint32_t kMaxLeafVisits = 1u << 22;
 
 // Root of one tree in a packed forest (many scenes in one node / reference /
 // triangle allocation): its node index and root box.
@@ -73,6 +76,7 @@ struct Trav {
     uint32_t node, root;
     uint32_t leaf_off, leaf_left;  // triangle references still to test in the current leaf
     int head, count, restarts;
+    uint32_t visits;  // leaves visited
     bool lost;
     uint32_t best_id;
     float best_r, best_s, best_t;
@@ -215,6 +219,7 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     T.head = 0;
     T.count = 0;
     T.restarts = 0;
+    T.visits = 0;
     T.lost = false;
     T.best_id = kInvalidId;
     T.best_r = FLT_MAX;
@@ -328,6 +333,10 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
         nd = sc.nodes[node];
     }
     if (COUNT) cnt.leaves++;
+    if (++T.visits > kMaxLeafVisits) {
+        cnt.incomplete = true;
+        return true;
+    }
     const uint32_t off = nd.x, nref = nd.y >> 2;
     for (uint32_t k = 0; k < nref; ++k) {
         const uint32_t id = sc.refs[off + k];
@@ -428,6 +437,10 @@ __device__ __forceinline__ bool trav_tick(Trav& T, const SceneView& sc, float tm
         T.texit = texit;
         if ((nd.y & 3u) != 3u) return false;  // still descending
         if (COUNT) cnt.leaves++;
+        if (++T.visits > kMaxLeafVisits) {
+            cnt.incomplete = true;
+            return true;
+        }
         T.leaf_off = nd.x;
         T.leaf_left = nd.y >> 2;
         if (T.leaf_left == 0) goto leaf_done;
@@ -609,8 +622,9 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const unsigned long long key = wslot[lane];
-        if (lcnt) {
-            if (key != ~0ull) {
+        // every active lane moves on after its leaf, empty leaves included
+        if (ray != kNone) {
+            if (lcnt && key != ~0ull) {
                 const uint32_t id = sc.refs[loff + (uint32_t)key];
                 float rr, ss, tt;
                 if (MT) leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
@@ -624,10 +638,17 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                     ray = kNone;
                 }
             }
-            if (ray != kNone && trav_after_leaf<S, EXH, ANY>(T, tmx, stk, stride, cnt)) {
-                if (COUNT) cnt.restarts += T.restarts;
-                store(ray, T, true);
-                ray = kNone;
+            if (ray != kNone) {
+                bool done = trav_after_leaf<S, EXH, ANY>(T, tmx, stk, stride, cnt);
+                if (!done && ++T.visits > kMaxLeafVisits) {
+                    cnt.incomplete = true;
+                    done = true;
+                }
+                if (done) {
+                    if (COUNT) cnt.restarts += T.restarts;
+                    store(ray, T, true);
+                    ray = kNone;
+                }
             }
         }
     }
