@@ -42,7 +42,7 @@ def main():
                     help="skip the guided-vs-plain time-to-relMSE measurement")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the wavefront render throughput measurement")
-    ap.add_argument("--guided-ref-spp", type=int, default=2048)
+    ap.add_argument("--guided-ref-spp", type=int, default=8192)
     ap.add_argument("--guided-width", type=int, default=512)
     ap.add_argument("--render-spp", type=int, default=2)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -211,7 +211,7 @@ def render_throughput(args):
     from turner_amd import scenes
     from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE
     sc = scenes.sphere_in_cornell(level=8)
-    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True)
     w, h = 1920, 1080
     conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
                           num_monte_carlo_samples=4, flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS, seed=1)
@@ -226,7 +226,7 @@ def render_throughput(args):
     return {
         "workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
                      "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
-                     f"{args.render_spp} pixel samples timed"),
+                     f"{args.render_spp} pixel samples timed, rendering kd-tree"),
         "closest_hit_mrays_s": round(st["rays"] / sec / 1e6, 1),
         "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
         "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 3),
@@ -251,13 +251,14 @@ def guided_vs_plain(args):
     from turner_amd.progressive import time_to_relmse
     from turner_amd.relmse import reference_image
     sc = scenes.occluded_light_cornell(level=8)
-    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True)
     w, depth = args.guided_width, 6
     flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
     t0 = time.perf_counter()
-    ref = reference_image(tree, sc, w, depth, flags, spp=args.guided_ref_spp, seed=777)
+    ref, ref_var = reference_image(tree, sc, w, depth, flags, spp=args.guided_ref_spp, seed=777, with_var=True)
     torch.cuda.synchronize()
     ref_s = time.perf_counter() - t0
+    ref_noise = float((ref_var[..., :3].double() / (ref[..., :3].double() ** 2 + 0.01)).mean())
     # warm-up of both paths (allocations), untimed
     g = Guide.for_tree(tree, seed=1)
     tiny = T.TracerConfig(width=w, max_recursion_depth=depth, num_pixel_samples=1, num_monte_carlo_samples=1,
@@ -266,9 +267,10 @@ def guided_vs_plain(args):
     T.render_accumulate(tree, sc["camera"], sc["lights"], tiny, scratch)
     render_guided(tree, sc["camera"], sc["lights"], tiny, g, scratch, train_samples=1 << 14)
     g.close()
-    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11)
-    g = Guide.for_tree(tree, seed=2)
-    guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12)
+    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=64)
+    g = Guide.for_tree(tree, seed=2, epochs=4)
+    guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12, ref_var=ref_var, spp_per_pass=64,
+                            train_passes=16, train_spp=2, train_samples=1 << 18, keep_training_passes=False)
     ginfo = g.info()
     # MFMA rate of the guide kernels (HIP events on the launch stream)
     dev = torch.device("cuda:0")
@@ -301,8 +303,13 @@ def guided_vs_plain(args):
     ratio = guided["seconds"] / plain["seconds"] if plain["reached"] and guided["reached"] else None
     return {
         "workload": (f"occluded-light Cornell box + level-8 icosphere, {w}x{w}, max-depth 6, cosine + "
-                     f"area-light NEE, relMSE target 0.01 vs a {args.guided_ref_spp}-spp reference "
-                     f"({ref_s:.1f} s)"),
+                     f"area-light NEE, rendering kd-tree, relMSE target 0.01 vs a {args.guided_ref_spp}-spp "
+                     f"plain reference ({ref_s:.1f} s; its own per-pixel variance, relMSE "
+                     f"{ref_noise:.4f}, is subtracted)"),
+        "guided_schedule": ("16 training passes of 2 spp, each training on a uniform 2^18-sample subset of "
+                            "its radiance records for 4 epochs (Adam lr 1e-3, batch 16384), then 64-spp "
+                            "passes; training passes are timed but left out of the image; one-sample MIS "
+                            "0.5 guide / 0.5 cosine"),
         "plain": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items() if k != "history"},
         "guided": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in guided.items() if k != "history"},
         "guided_time_over_plain": None if ratio is None else round(ratio, 4),

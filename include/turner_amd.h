@@ -264,9 +264,12 @@ int tn_guide_destroy(tn_guide* g);
  * set_params resets the Adam state. */
 int tn_guide_get_params(tn_guide* g, float* theta);
 int tn_guide_set_params(tn_guide* g, const float* theta);
-/* Adam learning rate (1e-3), mini-batch size (16384, a multiple of 64) and the
- * one-sample MIS probability of drawing from the guide (0.5). */
-int tn_guide_set_options(tn_guide* g, float lr, uint32_t batch, float mis_guided);
+/* Adam learning rate (1e-3), mini-batch size (16384, a multiple of 64), the
+ * one-sample MIS probability of drawing from the guide (0.5) and the epochs
+ * of each training pass (1): a pass trains on a uniform subset of
+ * train_samples of its recorded samples, epochs * train_samples / batch Adam
+ * steps. */
+int tn_guide_set_options(tn_guide* g, float lr, uint32_t batch, float mis_guided, uint32_t epochs);
 /* Network outputs (40 per point, device) for n points (device p3, n3). */
 int tn_guide_forward(tn_guide* g, const float* p3, const float* n3, uint32_t n, float* out,
                      void* stream);
@@ -300,8 +303,8 @@ long tn_guide_get_train_set(tn_guide* g, float* out, size_t cap);
  * mis_guided * q(w) + (1 - mis_guided) cos(w) / pi, so the estimator of
  * trace() stays unbiased.  With train_samples > 0 (needs mc_samples == 1)
  * the render also records, for every bounce, the radiance its path brought
- * back, and afterwards runs train_samples / batch Adam steps on mini-batches
- * drawn from those records.
+ * back, and afterwards trains on a uniform subset of train_samples of those
+ * records (epochs * train_samples / batch Adam steps, tn_guide_set_options).
  */
 int tn_render_guided(tn_scene* scene, const tn_camera* cam, const tn_light* lights,
                      uint32_t num_lights, const tn_render_params* params, tn_guide* guide,

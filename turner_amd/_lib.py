@@ -121,7 +121,7 @@ SIGNATURES = {
     "tn_guide_destroy": (C.c_int, [_P]),
     "tn_guide_get_params": (C.c_int, [_P, _P]),
     "tn_guide_set_params": (C.c_int, [_P, _P]),
-    "tn_guide_set_options": (C.c_int, [_P, C.c_float, _U32, C.c_float]),
+    "tn_guide_set_options": (C.c_int, [_P, C.c_float, _U32, C.c_float, _U32]),
     "tn_guide_forward": (C.c_int, [_P, _P, _P, _U32, _P, _P]),
     "tn_guide_pdf": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P]),
     "tn_guide_sample": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _P]),

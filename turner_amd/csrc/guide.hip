@@ -11,8 +11,8 @@
 //   gradient  dW_l = D_l^T H_{l-1}            K = the block's 64 samples
 //
 // Each block writes its gradient slice (and the sums of c and c * -log q)
-// to its own row of a partial buffer; the Adam kernel sums the rows in a
-// fixed order, so a step is deterministic.  The per-sample loss gradient
+// to its own row of a partial buffer; a two-stage reduction sums the rows in
+// a fixed order, so a step is deterministic.  The per-sample loss gradient
 // with respect to the 40 outputs is evaluated in FP32 in log space.
 #include <hip/hip_runtime.h>
 
@@ -226,6 +226,8 @@ struct TrainSrc {
     const uint32_t* ts_count;
     uint32_t ts_cap;
     uint64_t seed;
+    uint32_t subset;       // distinct samples this pass trains on
+    uint64_t subset_seed;  // fixes which ones
 };
 
 __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, float* __restrict__ partial) {
@@ -240,7 +242,10 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, flo
         if (S.ts) {
             const uint32_t cnt = min(*S.ts_count, S.ts_cap);
             if (cnt) {
-                const uint32_t k = (uint32_t)(g_splitmix64(S.seed ^ ((uint64_t)i << 20)) % cnt);
+                // position in this pass's subset of `subset` samples, then the
+                // subset's fixed uniform choice from the training set
+                const uint32_t j = (uint32_t)(g_splitmix64(S.seed ^ ((uint64_t)i << 20)) % S.subset);
+                const uint32_t k = (uint32_t)(g_splitmix64(S.subset_seed ^ ((uint64_t)j << 24)) % cnt);
                 const float4 a = S.ts[3 * (size_t)k], b = S.ts[3 * (size_t)k + 1], e = S.ts[3 * (size_t)k + 2];
                 p = make_float3(a.x, a.y, a.z);
                 c = a.w;
@@ -313,47 +318,80 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, flo
     bias_grad(sD[1], kGStrideHid, kGHid, P + kGOffB1);
 }
 
-__device__ __forceinline__ void batch_sums(const float* partial, uint32_t nb, float& sc, float& sl) {
-    sc = 0.f;
-    sl = 0.f;
-    for (uint32_t b = 0; b < nb; ++b) {
-        sc += partial[(size_t)b * kGPartial + kGParams];
-        sl += partial[(size_t)b * kGPartial + kGParams + 1];
+// Two-stage deterministic reduction of the per-block gradient slices:
+// stage 1 sums fixed groups of kGroupRows block rows into kGroups rows (and
+// reduces the per-block sums of c and c * nll into totals[0..1]); stage 2
+// (k_guide_reduce / k_guide_adam) sums the kGroups rows in order.
+constexpr uint32_t kGroups = 32;
+
+__global__ void k_guide_stage1(const float* __restrict__ partial, uint32_t nb, float* __restrict__ groups,
+                               float* __restrict__ totals) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = blockIdx.y;
+    const uint32_t per = (nb + kGroups - 1) / kGroups;
+    const uint32_t b0 = g * per, b1 = min(nb, b0 + per);
+    if (p < kGParams) {
+        float acc = 0.f;
+        for (uint32_t b = b0; b < b1; ++b) acc += partial[(size_t)b * kGPartial + p];
+        groups[(size_t)g * kGParams + p] = acc;
+    }
+    if (g == 0 && blockIdx.x == 0) {  // totals: one wave, lane-strided then a fixed shuffle tree
+        __shared__ float red[2][256];
+        float sc = 0.f, sl = 0.f;
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+            sc += partial[(size_t)b * kGPartial + kGParams];
+            sl += partial[(size_t)b * kGPartial + kGParams + 1];
+        }
+        red[0][threadIdx.x] = sc;
+        red[1][threadIdx.x] = sl;
+        __syncthreads();
+        for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + o];
+                red[1][threadIdx.x] += red[1][threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            totals[0] = red[0][0];
+            totals[1] = red[1][0];
+        }
     }
 }
 
+__device__ __forceinline__ float group_sum(const float* groups, uint32_t p) {
+    float g = 0.f;
+#pragma unroll 8
+    for (uint32_t k = 0; k < kGroups; ++k) g += groups[(size_t)k * kGParams + p];
+    return g;
+}
+
 // grad = (sum of the block slices) / (sum of c); loss = sum(c nll) / sum(c).
-__global__ void k_guide_reduce(const float* __restrict__ partial, uint32_t nb, float* __restrict__ grad,
-                               float* __restrict__ loss) {
+__global__ void k_guide_reduce(const float* __restrict__ groups, const float* __restrict__ totals,
+                               float* __restrict__ grad, float* __restrict__ loss) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    float sc, sl;
-    batch_sums(partial, nb, sc, sl);
+    const float sc = totals[0], sl = totals[1];
     if (p == 0) {
         loss[0] = sc > 0.f ? sl / sc : 0.f;
         loss[1] = sc;
     }
     if (p >= kGParams) return;
-    float g = 0.f;
-    for (uint32_t b = 0; b < nb; ++b) g += partial[(size_t)b * kGPartial + p];
-    grad[p] = sc > 0.f ? g / sc : 0.f;
+    grad[p] = sc > 0.f ? group_sum(groups, p) / sc : 0.f;
 }
 
 // Adam (Kingma & Ba 2015) with bias corrections bc1 = 1 - b1^t, bc2 = 1 - b2^t.
 // A batch whose weights sum to zero leaves every parameter unchanged.
 __global__ void k_guide_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
-                             const float* __restrict__ partial, uint32_t nb, float lr, float bc1,
-                             float bc2, float* __restrict__ loss) {
+                             const float* __restrict__ groups, const float* __restrict__ totals, float lr,
+                             float bc1, float bc2, float* __restrict__ loss) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    float sc, sl;
-    batch_sums(partial, nb, sc, sl);
+    const float sc = totals[0], sl = totals[1];
     if (p == 0) {
         loss[0] = sc > 0.f ? sl / sc : 0.f;
         loss[1] = sc;
     }
     if (p >= kGParams || !(sc > 0.f)) return;
-    float g = 0.f;
-    for (uint32_t b = 0; b < nb; ++b) g += partial[(size_t)b * kGPartial + p];
-    g /= sc;
+    const float g = group_sum(groups, p) / sc;
     const float mm = 0.9f * m[p] + 0.1f * g;
     const float vv = 0.999f * v[p] + 0.001f * g * g;
     m[p] = mm;
@@ -365,19 +403,37 @@ __global__ void k_zero_u32(uint32_t* p) {
     if (threadIdx.x == 0) *p = 0u;
 }
 
+uint64_t g_host_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
 int check_guide(tn_guide* g) {
     if (!g) return set_error(TN_ERR_INVALID, "NULL guide");
     TN_HIP(hipSetDevice(g->device));
     return TN_OK;
 }
 
+// Per-block slices, then kGroups group rows and two totals.
 int ensure_partial(tn_guide* g, uint32_t blocks) {
     if (g->partial_blocks >= blocks) return TN_OK;
     if (g->partial) (void)hipFree(g->partial);
     g->partial = nullptr;
     g->partial_blocks = 0;
-    TN_HIP(hipMalloc(&g->partial, (size_t)blocks * kGPartial * sizeof(float)));
+    TN_HIP(hipMalloc(&g->partial, ((size_t)blocks * kGPartial + (size_t)kGroups * kGParams + 64) * sizeof(float)));
     g->partial_blocks = blocks;
+    return TN_OK;
+}
+
+float* group_rows(tn_guide* g) { return g->partial + (size_t)g->partial_blocks * kGPartial; }
+float* group_totals(tn_guide* g) { return group_rows(g) + (size_t)kGroups * kGParams; }
+
+int reduce_stage1(tn_guide* g, uint32_t nb, hipStream_t stream) {
+    k_guide_stage1<<<dim3((kGParams + 255) / 256, kGroups), 256, 0, stream>>>(g->partial, nb, group_rows(g),
+                                                                             group_totals(g));
+    TN_HIP(hipGetLastError());
     return TN_OK;
 }
 
@@ -391,8 +447,10 @@ int adam_step(tn_guide* g, uint32_t nb, hipStream_t stream) {
     const uint32_t t = ++g->steps;
     const float bc1 = (float)(1.0 - std::pow(0.9, (double)t));
     const float bc2 = (float)(1.0 - std::pow(0.999, (double)t));
-    k_guide_adam<<<(kGParams + 255) / 256, 256, 0, stream>>>(g->theta, g->adam_m, g->adam_v, g->partial, nb,
-                                                             g->lr, bc1, bc2, g->dloss);
+    int rc = reduce_stage1(g, nb, stream);
+    if (rc) return rc;
+    k_guide_adam<<<(kGParams + 255) / 256, 256, 0, stream>>>(g->theta, g->adam_m, g->adam_v, group_rows(g),
+                                                             group_totals(g), g->lr, bc1, bc2, g->dloss);
     TN_HIP(hipGetLastError());
     return refresh(g, stream);
 }
@@ -422,13 +480,19 @@ int guide_train_from_set(tn_guide* g, uint32_t train_samples, hipStream_t stream
     const uint32_t nb = g->batch / 64;
     int rc = ensure_partial(g, nb);
     if (rc) return rc;
-    const uint32_t steps = std::max(1u, train_samples / g->batch);
+    // a uniform subset of train_samples recorded samples, `epochs` passes of
+    // mini-batches over it
+    const uint64_t draws = (uint64_t)train_samples * std::max(1u, g->epochs);
+    const uint32_t steps = (uint32_t)std::max<uint64_t>(1, draws / g->batch);
+    const uint64_t subset_seed = g_host_mix(g->seed ^ (0xA5A5A5A5ull + g->passes++));
     for (uint32_t s = 0; s < steps; ++s) {
         TrainSrc S{};
         S.n = g->batch;
         S.ts = g->ts;
         S.ts_count = g->ts_count;
         S.ts_cap = g->ts_cap;
+        S.subset = std::max(1u, train_samples);
+        S.subset_seed = subset_seed;
         S.seed = g->seed * 0x9E3779B97F4A7C15ull + ((uint64_t)g->steps << 32) + s;
         k_guide_train<<<nb, 64, 0, stream>>>(g->view(), S, g->partial);
         TN_HIP(hipGetLastError());
@@ -531,16 +595,18 @@ int tn_guide_set_params(tn_guide* g, const float* theta) {
     return TN_OK;
 }
 
-int tn_guide_set_options(tn_guide* g, float lr, uint32_t batch, float mis_guided) {
+int tn_guide_set_options(tn_guide* g, float lr, uint32_t batch, float mis_guided, uint32_t epochs) {
     if (!g) return set_error(TN_ERR_INVALID, "NULL guide");
     if (!(lr > 0.f) || !std::isfinite(lr)) return set_error(TN_ERR_INVALID, "learning rate must be > 0");
     if (batch == 0 || batch % 64 || batch > (1u << 24))
         return set_error(TN_ERR_INVALID, "batch must be a positive multiple of 64 up to 2^24");
     if (!(mis_guided >= 0.f && mis_guided <= 1.f))
         return set_error(TN_ERR_INVALID, "mis_guided must be in [0, 1]");
+    if (epochs == 0 || epochs > 1024) return set_error(TN_ERR_INVALID, "epochs must be in 1..1024");
     g->lr = lr;
     g->batch = batch;
     g->mis_guided = mis_guided;
+    g->epochs = epochs;
     return TN_OK;
 }
 
@@ -592,10 +658,11 @@ int tn_guide_loss_grad(tn_guide* g, const float* p3, const float* n3, const floa
     const uint32_t nb = (n + 63) / 64;
     if ((rc = ensure_partial(g, nb))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    TrainSrc S{p3, n3, w3, c, n, nullptr, nullptr, 0, 0};
+    TrainSrc S{p3, n3, w3, c, n, nullptr, nullptr, 0, 0, 1, 0};
     k_guide_train<<<nb, 64, 0, s>>>(g->view(), S, g->partial);
     TN_HIP(hipGetLastError());
-    k_guide_reduce<<<(kGParams + 255) / 256, 256, 0, s>>>(g->partial, nb, grad, g->dloss);
+    if ((rc = reduce_stage1(g, nb, s))) return rc;
+    k_guide_reduce<<<(kGParams + 255) / 256, 256, 0, s>>>(group_rows(g), group_totals(g), grad, g->dloss);
     TN_HIP(hipGetLastError());
     if (loss) {
         TN_HIP(hipMemcpyAsync(loss, g->dloss, sizeof(float), hipMemcpyDeviceToHost, s));
@@ -613,7 +680,7 @@ int tn_guide_train_step(tn_guide* g, const float* p3, const float* n3, const flo
     const uint32_t nb = (n + 63) / 64;
     if ((rc = ensure_partial(g, nb))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    TrainSrc S{p3, n3, w3, c, n, nullptr, nullptr, 0, 0};
+    TrainSrc S{p3, n3, w3, c, n, nullptr, nullptr, 0, 0, 1, 0};
     k_guide_train<<<nb, 64, 0, s>>>(g->view(), S, g->partial);
     TN_HIP(hipGetLastError());
     if ((rc = adam_step(g, nb, s))) return rc;

@@ -243,6 +243,8 @@ struct tn_guide {
     float lr = 1e-3f;
     uint32_t batch = 16384;
     float mis_guided = 0.5f;
+    uint32_t epochs = 1;   // passes over each training pass's subset of samples
+    uint32_t passes = 0;   // training passes so far
     uint32_t last_samples = 0;
 
     tn::GuideView view() const {

@@ -40,13 +40,13 @@ def _dev(t, cols=None):
 class Guide:
     """tn_guide: parameters, Adam state and training set on the device."""
 
-    def __init__(self, bmin, bmax, seed=1, lr=1e-3, batch=16384, mis_guided=0.5):
+    def __init__(self, bmin, bmax, seed=1, lr=1e-3, batch=16384, mis_guided=0.5, epochs=1):
         h = C.c_void_p()
         bmin = np.ascontiguousarray(bmin, np.float32)
         bmax = np.ascontiguousarray(bmax, np.float32)
         check(lib().tn_guide_create(ptr(bmin), ptr(bmax), seed, C.byref(h)))
         self.handle = h
-        self.set_options(lr, batch, mis_guided)
+        self.set_options(lr, batch, mis_guided, epochs)
 
     @classmethod
     def for_tree(cls, tree, **kw):
@@ -64,9 +64,9 @@ class Guide:
         except Exception:
             pass
 
-    def set_options(self, lr=1e-3, batch=16384, mis_guided=0.5):
-        check(lib().tn_guide_set_options(self.handle, lr, batch, mis_guided))
-        self.lr, self.batch, self.mis_guided = lr, batch, mis_guided
+    def set_options(self, lr=1e-3, batch=16384, mis_guided=0.5, epochs=1):
+        check(lib().tn_guide_set_options(self.handle, lr, batch, mis_guided, epochs))
+        self.lr, self.batch, self.mis_guided, self.epochs = lr, batch, mis_guided, epochs
 
     def params(self):
         theta = np.zeros(GUIDE_PARAMS, np.float32)

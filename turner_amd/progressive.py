@@ -24,9 +24,9 @@ def _conf(width, height, depth, flags, spp, seed):
                         num_monte_carlo_samples=1, flags=flags, seed=seed)
 
 
-def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=16,
-                   train_passes=16, train_samples=1 << 18, train_spp=1, max_seconds=60.0, seed=1,
-                   max_spp=1 << 16, ref_var=None, keep_training_passes=True, train_every=0):
+def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=64,
+                   train_passes=16, train_samples=1 << 18, train_spp=2, max_seconds=60.0, seed=1,
+                   max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
     reference's per-pixel variance (relmse subtracts it).
