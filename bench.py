@@ -42,9 +42,14 @@ def main():
                     help="skip the guided-vs-plain time-to-relMSE measurement")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the wavefront render throughput measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched heterogeneous rendering measurement")
     ap.add_argument("--guided-ref-spp", type=int, default=8192)
     ap.add_argument("--guided-width", type=int, default=512)
     ap.add_argument("--render-spp", type=int, default=2)
+    ap.add_argument("--mlp-probe", action="store_true",
+                    help="only the guide MLP kernels: 3 forward launches over 2^20 points and 3 "
+                         "training steps of 2^16 samples (for rocprofv3 MFMA counters), then exit")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one traversal launch plus a 1 GiB device copy (the calibration "
                          "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE), then exit")
@@ -61,6 +66,9 @@ def main():
     from turner_amd import scenes
     from turner_amd._lib import TN_ISECT_COUNT
 
+    if args.mlp_probe:
+        print(json.dumps(mlp_probe()), flush=True)
+        return
     dev = torch.device("cuda:0")
     name, cus = T.device_info()
     sc = scenes.intersect_scene(level=args.level)
@@ -194,9 +202,116 @@ def main():
         }
     if not args.no_render:
         out["render"] = render_throughput(args)
+    if not args.no_batch:
+        out["batch"] = batch_throughput(args)
     if not args.no_guided:
         out["guided"] = guided_vs_plain(args)
     print(json.dumps(out), flush=True)
+
+
+def batch_throughput(args):
+    """Batched heterogeneous rendering: 32 scenes in one packed forest and one
+    launch sequence per chunk.  Scene 0 is the white furnace (enclosure
+    emitting 1, albedo-0.8 sphere); scenes 1..31 are seeded displaced
+    icospheres of subdivision level uniform in 3..8 (1,280 to 1.31M
+    triangles, log-uniform) in the Cornell enclosure with a ceiling emitter;
+    square images of width uniform in 128..1024, 16..64 pixel samples,
+    max-depth 3, one path per sample, cosine + area-light NEE; chunk size and
+    grid from the autotuner (probe excluded from the timed render)."""
+    import numpy as np
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_AUTOTUNE, TN_RENDER_COSINE, TN_RENDER_GRAPH
+    from turner_amd.batch import Batch
+    rng = np.random.default_rng(2024)
+    scs = [scenes.furnace(level=4, rho=0.8, radiance=1.0)]
+    levels = [4]
+    for k in range(1, 32):
+        lv = int(rng.integers(3, 9))
+        levels.append(lv)
+        scs.append(scenes.sphere_in_cornell(level=lv, seed=k))
+    t0 = time.perf_counter()
+    trees = [T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True) for sc in scs]
+    build_s = time.perf_counter() - t0
+    jobs = []
+    for sc in scs:
+        w = int(rng.integers(128, 1025))
+        jobs.append({"camera": sc["camera"], "lights": sc["lights"], "width": w, "height": w,
+                     "spp": int(rng.integers(16, 65))})
+    b = Batch(trees)
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS | TN_RENDER_GRAPH | TN_RENDER_AUTOTUNE
+    # tuning call (probes + one full render), then the timed render with the cached shape
+    b.render(jobs, max_depth=3, mc_samples=1, flags=flags, seed=1)
+    tune = b.tuning()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    imgs, st, _ = b.render(jobs, max_depth=3, mc_samples=1, flags=flags, seed=2, sample_offset=1 << 20)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    # furnace check: pixels whose four corner rays hit the sphere see rho * L = 0.8,
+    # pixels whose corners all miss it see the enclosure, exactly 1
+    f = imgs[0].cpu().numpy()
+    w0 = jobs[0]["width"]
+    cam = scs[0]["camera"]
+    ys, xs = np.mgrid[0:w0 + 1, 0:w0 + 1]
+    cx = -cam["delta_x"] * (1 - 2 * xs / w0)
+    cy = cam["delta_y"] * (1 - 2 * ys / w0)
+    d = (np.stack([cx, cy, -np.ones_like(cx)], -1).reshape(-1, 3) @ np.asarray(cam["trafo"]).T).astype(np.float32)
+    o = np.tile(np.asarray(cam["pos"], np.float32), (len(d), 1))
+    ids = T.KDTreeIntersection(trees[0]).intersect_batch(o, d)[0].reshape(w0 + 1, w0 + 1)
+    nsph = 20 * 4 ** 4
+    on = (ids[:-1, :-1] < nsph) & (ids[1:, :-1] < nsph) & (ids[:-1, 1:] < nsph) & (ids[1:, 1:] < nsph)
+    off = (ids[:-1, :-1] >= nsph) & (ids[1:, :-1] >= nsph) & (ids[:-1, 1:] >= nsph) & (ids[1:, 1:] >= nsph)
+    sph = f[on][:, 0]
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    b.close()
+    return {
+        "workload": ("32 scenes in one batch: white furnace + 31 displaced icospheres (levels 3..8) in "
+                     "Cornell enclosures, widths 128..1024, 16..64 spp, max-depth 3, m 1, cosine + "
+                     "area-light NEE, rendering kd-trees, graph-replayed chunks, autotuned shape"),
+        "scenes": len(scs), "levels": levels, "triangles": int(sum(len(sc["verts"]) for sc in scs)),
+        "samples": samples, "host_build_s": round(build_s, 2), "seconds": round(sec, 3),
+        "closest_hit_mrays_s": round(st["rays"] / sec / 1e6, 1),
+        "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
+        "tuning": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in tune.items()},
+        "furnace": {"sphere_mean": round(float(sph.mean()), 5),
+                    "sphere_sem": round(float(sph.std() / np.sqrt(max(sph.size, 1))), 5),
+                    "expected": 0.8, "sphere_pixels": int(on.sum()), "spp": jobs[0]["spp"],
+                    "enclosure_max_abs_err": round(float(np.abs(f[off][:, :3] - 1.0).max()), 7)},
+    }
+
+
+def mlp_probe():
+    """Guide MLP kernels alone: forward over 2^20 points, training steps of
+    2^16 samples, timed with HIP events on the launch stream."""
+    import torch
+
+    from turner_amd.guide import Guide
+    g = Guide([0.0, 0.0, 0.0], [1.0, 1.0, 1.0], seed=1)
+    n, nb = 1 << 20, 1 << 16
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    p = torch.rand((n, 3), device="cuda", generator=gen)
+    nrm = torch.nn.functional.normalize(torch.randn((n, 3), device="cuda", generator=gen), dim=1)
+    w = torch.nn.functional.normalize(torch.randn((n, 3), device="cuda", generator=gen), dim=1)
+    c = torch.rand(n, device="cuda", generator=gen)
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, fn, flop in (("forward", lambda: g.forward(p, nrm), 31488.0 * n),
+                           ("train_step", lambda: g.train_step(p[:nb], nrm[:nb], w[:nb], c[:nb], want_loss=False),
+                            3 * 31488.0 * nb)):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(3):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        out[name] = {"ms": round(ms, 4), "tflops": round(flop / (ms * 1e-3) / 1e12, 2)}
+    g.close()
+    return out
 
 
 def render_throughput(args):
