@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--mlp-probe", action="store_true",
                     help="only the guide MLP kernels: 3 forward launches over 2^20 points and 3 "
                          "training steps of 2^16 samples (for rocprofv3 MFMA counters), then exit")
+    ap.add_argument("--tree-probe", action="store_true",
+                    help="reference vs rendering kd-tree on render-like ray batches of the "
+                         "occluded-light scene (camera, shadow, bounce), with work counters, then exit")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one traversal launch plus a 1 GiB device copy (the calibration "
                          "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE), then exit")
@@ -68,6 +71,9 @@ def main():
 
     if args.mlp_probe:
         print(json.dumps(mlp_probe()), flush=True)
+        return
+    if args.tree_probe:
+        print(json.dumps(tree_probe()), flush=True)
         return
     dev = torch.device("cuda:0")
     name, cus = T.device_info()
@@ -281,6 +287,100 @@ def batch_throughput(args):
                     "expected": 0.8, "sphere_pixels": int(on.sum()), "spp": jobs[0]["spp"],
                     "enclosure_max_abs_err": round(float(np.abs(f[off][:, :3] - 1.0).max()), 7)},
     }
+
+
+def tree_probe():
+    """tn_intersect / tn_occluded on render-like batches of the occluded-light
+    scene (512x512 at 16 spp): camera rays, shadow rays from the camera hits to
+    uniform emitter points, cosine bounces from the camera hits; reference kd-
+    tree vs rendering kd-tree; HIP-event times and per-ray work counters."""
+    import numpy as np
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_ISECT_COUNT
+    sc = scenes.occluded_light_cornell(level=8)
+    rng = np.random.default_rng(1)
+    w, spp = 512, 16
+    cam = sc["camera"]
+    ys, xs = np.mgrid[0:w, 0:w]
+    xs = np.repeat(xs.ravel(), spp) + rng.uniform(size=w * w * spp)
+    ys = np.repeat(ys.ravel(), spp) + rng.uniform(size=w * w * spp)
+    cx = -cam["delta_x"] * (1 - 2 * xs / w)
+    cy = cam["delta_y"] * (1 - 2 * ys / w)
+    cd = (np.stack([cx, cy, -np.ones_like(cx)], -1) @ np.asarray(cam["trafo"]).T).astype(np.float32)
+    co = np.tile(np.asarray(cam["pos"], np.float32), (len(cd), 1))
+    verts = sc["verts"].reshape(-1, 3, 3)
+    L = T.lib()
+    out = {}
+    batches = None
+    for label, el in (("reference", False), ("rendering", True)):
+        tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=el)
+
+        def run(o, d, tmax=None):
+            n = len(o)
+            to, td = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+            ids = torch.empty(n, dtype=torch.int32, device="cuda")
+            r, s_, t_ = (torch.empty(n, device="cuda") for _ in range(3))
+            tm = torch.from_numpy(tmax).cuda() if tmax is not None else None
+            occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+            st = torch.cuda.current_stream().cuda_stream
+
+            def go(f):
+                if tm is None:
+                    T.check(L.tn_intersect(tree.handle, T.ptr(to), T.ptr(td), n, T.ptr(ids), T.ptr(r), T.ptr(s_),
+                                           T.ptr(t_), f, st))
+                else:
+                    T.check(L.tn_occluded(tree.handle, T.ptr(to), T.ptr(td), T.ptr(tm), n, T.ptr(occ), f, st))
+            go(0)
+            cnt = np.zeros(8, np.uint64)
+            T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
+            go(TN_ISECT_COUNT)
+            torch.cuda.synchronize()
+            T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            go(0)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            c = cnt.astype(np.float64) / n
+            res = {"n": n, "ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1)}
+            if tm is None:
+                res.update({"inner_per_ray": round(c[0], 2), "leaves_per_ray": round(c[1], 2),
+                            "tests_per_ray": round(c[3], 2)})
+            return res, ids.cpu().numpy().view(np.uint32), r.cpu().numpy()
+
+        res_cam, ids, r = run(co, cd)
+        if batches is None:  # secondary batches from the reference tree's camera hits
+            hit = ids != T.TN_INVALID_ID
+            p = co[hit] + r[hit, None] * cd[hit]
+            v = verts[ids[hit]]
+            nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+            nrm *= -np.sign(np.sum(nrm * cd[hit], axis=1))[:, None]
+            p2 = (p + 1e-4 * nrm).astype(np.float32)
+            q = np.stack([0.35 + 0.3 * rng.uniform(size=len(p)), np.full(len(p), 0.999),
+                          0.35 + 0.3 * rng.uniform(size=len(p))], 1)
+            ld = q - p2
+            dist = np.linalg.norm(ld, axis=1)
+            ld /= dist[:, None]
+            u1, u2 = rng.uniform(size=len(p)), rng.uniform(size=len(p))
+            a = np.where(np.abs(nrm[:, 0:1]) > 0.5, np.array([[0.0, 1, 0]]), np.array([[1.0, 0, 0]]))
+            b1 = np.cross(nrm, a)
+            b1 /= np.linalg.norm(b1, axis=1, keepdims=True)
+            b2 = np.cross(nrm, b1)
+            rr, ph = np.sqrt(u1), 2 * np.pi * u2
+            bd = (rr * np.cos(ph))[:, None] * b1 + (rr * np.sin(ph))[:, None] * b2 + np.sqrt(1 - u1)[:, None] * nrm
+            batches = (p2, ld.astype(np.float32), (dist * (1 - 1e-4)).astype(np.float32), bd.astype(np.float32))
+        p2, ld, tmx, bd = batches
+        res_sh, _, _ = run(p2, ld, tmx)
+        res_b, _, _ = run(p2, bd)
+        out[label] = {"camera": res_cam, "shadow_any_hit": res_sh, "bounce": res_b,
+                      "device_nodes": tree.num_nodes(), "height": tree.height()}
+        tree.close()
+    return out
 
 
 def mlp_probe():
