@@ -110,6 +110,47 @@ __global__ __launch_bounds__(64) void k_guide_eval(GuideView G, const float* __r
     pdf[i] = mixture_pdf(M, w);
 }
 
+// MLP outputs for n points with persistent waves: each wave (block) loads
+// every BF16 weight fragment into registers once and then runs groups of 64
+// points, so weights are not re-read per group.
+__global__ __launch_bounds__(64) void k_guide_forward(GuideView G, const float* __restrict__ p3,
+                                                      const float* __restrict__ n3, uint32_t n,
+                                                      float* __restrict__ out) {
+    __shared__ __align__(16) uint16_t lds[64 * (kGStrideIn + kGStrideHid)];
+    g_s4 w1[4][5], w2[4][4], w3[4][4], w4[3][4];
+    load_frags<5, 4>(G.wbf + kGW1, w1);
+    load_frags<4, 4>(G.wbf + kGW2, w2);
+    load_frags<4, 4>(G.wbf + kGW3, w3);
+    load_frags<4, 3>(G.wbf + kGW4, w4);
+    const uint32_t l = threadIdx.x;
+    uint16_t* a = lds;
+    uint16_t* b = lds + 64 * kGStrideIn;
+    float* o = reinterpret_cast<float*>(lds);
+    const uint32_t groups = (n + 63) / 64;
+    for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const uint32_t base = grp * 64u, i = base + l;
+        const bool act = i < n;
+        float3 p = make_float3(0.f, 0.f, 0.f), nn = p;
+        if (act) {
+            p = load3(p3, i);
+            nn = load3(n3, i);
+        }
+        guide_features(G, p, nn, act, a + l * kGStrideIn);
+        __syncthreads();
+        mfma_wx_reg<5, 4>(w1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
+        __syncthreads();
+        mfma_wx_reg<4, 4>(w2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
+        __syncthreads();
+        mfma_wx_reg<4, 4>(w3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
+        __syncthreads();
+        mfma_wx_reg<4, 3>(w4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, o});
+        __syncthreads();
+        const uint32_t rows = min(64u, n - base);
+        for (uint32_t k = l; k < rows * kGOut; k += 64) out[(size_t)base * kGOut + k] = o[k];
+        __syncthreads();
+    }
+}
+
 // d(c * -log q(w)) / d(out) for one sample, FP32 in log space; returns -log q.
 __device__ float guide_dout(const float* o, float3 w, float c, float* d) {
     const float kLog2Pi = 1.83787706640934548356f;
@@ -209,8 +250,13 @@ __device__ __forceinline__ void bias_grad(const uint16_t* D, int ds, int out_rea
 template <int K_TILES>
 __device__ __forceinline__ void delta_back(const uint16_t* WT, const uint16_t* D, const uint16_t* H,
                                            uint16_t* dst) {
-    mfma_wx<K_TILES, 4>(WT, D, kGStrideHid, [&](int o, int smp, float v) {
-        dst[smp * kGStrideHid + o] = from_bf16(H[smp * kGStrideHid + o]) > 0.f ? to_bf16(v) : (uint16_t)0;
+    mfma_wx<K_TILES, 4>(WT, D, kGStrideHid, [&](int o, int smp, g_f4 v) {
+        const uint2 h = *reinterpret_cast<const uint2*>(H + smp * kGStrideHid + o);
+        const float m0 = from_bf16((uint16_t)h.x) > 0.f ? v[0] : 0.f;
+        const float m1 = from_bf16((uint16_t)(h.x >> 16)) > 0.f ? v[1] : 0.f;
+        const float m2 = from_bf16((uint16_t)h.y) > 0.f ? v[2] : 0.f;
+        const float m3 = from_bf16((uint16_t)(h.y >> 16)) > 0.f ? v[3] : 0.f;
+        *reinterpret_cast<uint2*>(dst + smp * kGStrideHid + o) = make_uint2(pack_bf16x2(m0, m1), pack_bf16x2(m2, m3));
     });
 }
 
@@ -616,8 +662,11 @@ int tn_guide_forward(tn_guide* g, const float* p3, const float* n3, uint32_t n, 
     if (n && (!p3 || !n3 || !out)) return set_error(TN_ERR_INVALID, "NULL argument");
     if (n > (1u << 30)) return set_error(TN_ERR_LIMIT, "too many points");
     if (!n) return TN_OK;
-    k_guide_eval<<<(n + 63) / 64, 64, 0, (hipStream_t)stream>>>(g->view(), p3, n3, nullptr, nullptr, n, 0, out,
-                                                                nullptr, nullptr);
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
+    const uint32_t groups = (n + 63) / 64;
+    const uint32_t grid = std::min<uint32_t>(groups, (uint32_t)cus * 8);  // resident: 8 one-wave blocks per CU
+    k_guide_forward<<<grid, 64, 0, (hipStream_t)stream>>>(g->view(), p3, n3, n, out);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }

@@ -154,9 +154,9 @@ __device__ __forceinline__ void guide_features(const GuideView& G, float3 p, flo
 // mfma_f32_16x16x16bf16_1k.  A = W tile (lane: row 16mt + l%16, columns
 // 16kt + 4(l/16) .. +3 of a row-major [16 OUT_TILES][16 IN_TILES] matrix),
 // B = X^T tile read from the [sample][feature] LDS rows (lane: sample
-// 16nt + l%16, features 16kt + 4(l/16) .. +3).  The accumulator element r of
-// lane l is output 16mt + 4(l/16) + r of sample 16nt + l%16, handed to
-// epi(output, sample, value).
+// 16nt + l%16, features 16kt + 4(l/16) .. +3).  The accumulator of lane l
+// holds outputs 16mt + 4(l/16) .. +3 of sample 16nt + l%16 -- four
+// consecutive features of one row -- handed to epi(first output, sample, v).
 template <int IN_TILES, int OUT_TILES, class Epi>
 __device__ __forceinline__ void mfma_wx(const uint16_t* __restrict__ W, const uint16_t* in, int in_stride,
                                         Epi epi) {
@@ -177,27 +177,88 @@ __device__ __forceinline__ void mfma_wx(const uint16_t* __restrict__ W, const ui
                     *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfrag[kt], acc, 0, 0, 0);
             }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) epi(16 * mt + 4 * grp + r, 16 * nt + row, acc[r]);
+            epi(16 * mt + 4 * grp, 16 * nt + row, acc);
         }
     }
 }
+
+// Same product with the A fragments already in registers (Wf[mt][kt]).
+template <int IN_TILES, int OUT_TILES, class Epi>
+__device__ __forceinline__ void mfma_wx_reg(const g_s4 (&Wf)[OUT_TILES][IN_TILES], const uint16_t* in,
+                                            int in_stride, Epi epi) {
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        g_s4 bfrag[IN_TILES];
+#pragma unroll
+        for (int kt = 0; kt < IN_TILES; ++kt)
+            bfrag[kt] = *reinterpret_cast<const g_s4*>(in + (16 * nt + row) * in_stride + 16 * kt + 4 * grp);
+#pragma unroll
+        for (int mt = 0; mt < OUT_TILES; ++mt) {
+            g_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < IN_TILES; ++kt)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wf[mt][kt], bfrag[kt], acc, 0, 0, 0);
+            epi(16 * mt + 4 * grp, 16 * nt + row, acc);
+        }
+    }
+}
+
+// Loads the A fragments of a [16 OUT_TILES][16 IN_TILES] BF16 matrix.
+template <int IN_TILES, int OUT_TILES>
+__device__ __forceinline__ void load_frags(const uint16_t* __restrict__ W, g_s4 (&Wf)[OUT_TILES][IN_TILES]) {
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int mt = 0; mt < OUT_TILES; ++mt)
+#pragma unroll
+        for (int kt = 0; kt < IN_TILES; ++kt)
+            Wf[mt][kt] = *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
+}
+
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
+}
+
+// Epilogue of a hidden layer: bias, ReLU, BF16, one 8-byte LDS store of the
+// four consecutive features.
+struct HiddenEpi {
+    const float* bias;
+    uint16_t* out;
+    int stride;
+    __device__ __forceinline__ void operator()(int o, int smp, g_f4 v) const {
+        const float4 b = *reinterpret_cast<const float4*>(bias + o);
+        uint2 w;
+        w.x = pack_bf16x2(fmaxf(v[0] + b.x, 0.f), fmaxf(v[1] + b.y, 0.f));
+        w.y = pack_bf16x2(fmaxf(v[2] + b.z, 0.f), fmaxf(v[3] + b.w, 0.f));
+        *reinterpret_cast<uint2*>(out + smp * stride + o) = w;
+    }
+};
+
+// Epilogue of the output layer: 40 FP32 outputs per sample at out[smp * 40].
+struct OutputEpi {
+    const float* bias;
+    float* out;
+    __device__ __forceinline__ void operator()(int o, int smp, g_f4 v) const {
+        if (o >= kGOut) return;  // padded rows 40..47
+        const float4 b = *reinterpret_cast<const float4*>(bias + o);
+        *reinterpret_cast<float4*>(out + smp * kGOut + o) =
+            make_float4(v[0] + b.x, v[1] + b.y, v[2] + b.z, v[3] + b.w);
+    }
+};
 
 // Hidden layer: bias, ReLU, BF16 into the next [sample][feature] rows.
 template <int IN_TILES>
 __device__ __forceinline__ void guide_hidden(const uint16_t* W, const float* bias, const uint16_t* in,
                                              int in_stride, uint16_t* out, int out_stride) {
-    mfma_wx<IN_TILES, 4>(W, in, in_stride, [&](int o, int smp, float v) {
-        out[smp * out_stride + o] = to_bf16(fmaxf(v + bias[o], 0.f));
-    });
+    mfma_wx<IN_TILES, 4>(W, in, in_stride, HiddenEpi{bias, out, out_stride});
 }
 
 // Output layer: 40 FP32 outputs per sample into out[sample * 40 + o].
 __device__ __forceinline__ void guide_output(const uint16_t* W, const float* bias, const uint16_t* in,
                                              int in_stride, float* out) {
-    mfma_wx<4, 3>(W, in, in_stride, [&](int o, int smp, float v) {
-        if (o < kGOut) out[smp * kGOut + o] = v + bias[o];
-    });
+    mfma_wx<4, 3>(W, in, in_stride, OutputEpi{bias, out});
 }
 
 // Runs the MLP for the 64 lanes' samples of one wave (the block must be that
