@@ -75,6 +75,27 @@ def test_cli_renders_cornell(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_guided_render_tree(gpu, tmp_path):
+    """--guided (online-trained guide, then guided passes) on the rendering
+    tree: the image has the plain tracer's statistics."""
+    sc = scenes.sphere_in_cornell(level=3)
+    path = str(tmp_path / "sic.tns")
+    scenes.save_tns(sc, path)
+    common = ["-w", "32", "-d", "3", "-m", "1", "-p", "256", "--cosine", "--area-lights", "--render-tree",
+              "--no-gamma-correction"]
+    g = run(path, *common, "--guided", "--train-passes", "4")
+    assert g.returncode == 0, g.stderr
+    q = run(path, *common)
+    assert q.returncode == 0, q.stderr
+    pg = np.array([int(x) for x in " ".join(g.stdout.strip().split("\n")[3:]).split()], float)
+    pq = np.array([int(x) for x in " ".join(q.stdout.strip().split("\n")[3:]).split()], float)
+    assert pg.size == pq.size == 32 * 32 * 3
+    print(f"[cli guided] mean {pg.mean():.2f} vs plain {pq.mean():.2f}", flush=True)
+    assert abs(pg.mean() - pq.mean()) < 0.03 * pq.mean() + 1.0
+    assert run(path, "-w", "8", "-m", "2", "--guided").returncode != 0  # guiding needs -m 1
+
+
+@pytest.mark.gpu
 def test_trace_batch_matches_render(gpu):
     """tn_trace (the batched trace()) at max_depth 0 is deterministic and
     counts one segment per ray; a start depth beyond max_depth traces nothing."""

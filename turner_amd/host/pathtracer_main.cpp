@@ -46,6 +46,11 @@ MI355X options:
   --ref-jitter                      Pixel jitter of the reference (xorshift64*, seed 42 per row).
   --exhaustive                      Reference traversal order without early exit.
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
+  --render-tree                     Rendering kd-tree (drops the reference builder's fully clipped
+                                    triangles and empty-side quirks; same closest hits).
+  --graph                           Replay the per-chunk launch sequence from a hipGraph.
+  --guided                          Learned path guiding (needs -m 1): trains online, then renders.
+  --train-passes=<int>              Guiding: training passes of 2 spp [default: 16].
   --print-config                    Print the parsed configuration and exit.
 )";
 
@@ -59,7 +64,8 @@ const std::map<std::string, std::string> kShort = {
     {"-w", "--width"}, {"-a", "--aspect"}, {"-t", "--threads"}, {"-d", "--max-depth"},
     {"-p", "--pixel-samples"}, {"-m", "--monte-carlo-samples"}, {"-v", "--verbose"}};
 const char* kFlags[] = {"--no-gamma-correction", "--verbose", "--cosine", "--area-lights",
-                        "--ref-jitter", "--exhaustive", "--print-config", "--help"};
+                        "--ref-jitter", "--exhaustive", "--print-config", "--help",
+                        "--render-tree", "--graph", "--guided"};
 
 bool is_flag(const std::string& name) {
     for (const char* f : kFlags)
@@ -120,6 +126,8 @@ std::string get(const Args& a, const std::string& k, const std::string& def) {
 int main(int argc, char** argv) {
     using namespace turner_amd;
     TracerConfig conf;
+    bool render_tree = false, guided = false;
+    int train_passes = 16;
     try {
         Args a = parse(argc, argv);
         if (a.opt.count("--help") || a.pos.size() != 1) {
@@ -144,6 +152,12 @@ int main(int argc, char** argv) {
         if (a.opt.count("--area-lights")) conf.flags |= TN_RENDER_AREA_LIGHTS;
         if (a.opt.count("--ref-jitter")) conf.flags |= TN_RENDER_REF_JITTER;
         if (a.opt.count("--exhaustive")) conf.flags |= TN_RENDER_EXHAUSTIVE;
+        if (a.opt.count("--graph")) conf.flags |= TN_RENDER_GRAPH;
+        render_tree = a.opt.count("--render-tree") > 0;
+        guided = a.opt.count("--guided") > 0;
+        train_passes = std::stoi(get(a, "--train-passes", "16"));
+        if (guided && (conf.num_monte_carlo_samples != 1 || train_passes < 0))
+            throw std::runtime_error("--guided needs -m 1 and --train-passes >= 0");
         // config.h:29-33,119-126
         if (!(0 < conf.aspect && 1 <= conf.num_threads && 0 <= conf.exposure &&
               0 < conf.max_recursion_depth && 1 <= conf.num_pixel_samples &&
@@ -173,13 +187,22 @@ int main(int argc, char** argv) {
         float aspect = sc.camera.aspect > 0 ? sc.camera.aspect : conf.aspect;
         std::cerr << "Loading triangles and building kd-tree..." << std::endl;
         const size_t ntri = sc.triangles.size();
-        KDTree tree = conf.tree_cache.empty() ? KDTree(std::move(sc.triangles))
-                                              : KDTree(std::move(sc.triangles), conf.tree_cache);
+        KDTree tree = render_tree ? KDTree(std::move(sc.triangles), (uint32_t)TN_SCENE_EMPTY_LEAVES)
+                      : conf.tree_cache.empty() ? KDTree(std::move(sc.triangles))
+                                                : KDTree(std::move(sc.triangles), conf.tree_cache);
         auto t1 = std::chrono::steady_clock::now();
         std::cerr << "KDTree runtime: " << (size_t)tree.build_ms() << std::endl;
         const size_t height = (size_t)(conf.width / aspect);
         std::cerr << "Rendering ";
-        RenderResult res = render(tree, sc.camera, sc.lights, conf, height);
+        RenderResult res;
+        if (guided) {
+            Guide g(tree, conf.seed);
+            GuidedSchedule sched;
+            sched.train_passes = train_passes;
+            res = render_guided(tree, sc.camera, sc.lights, conf, height, g, sched);
+        } else {
+            res = render(tree, sc.camera, sc.lights, conf, height);
+        }
         std::cerr << std::endl;
         const double load_s = std::chrono::duration<double>(t1 - t0).count();
         const double ms = res.stats.ms;

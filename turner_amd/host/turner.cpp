@@ -107,14 +107,18 @@ KDTree::KDTree(Triangles tris, const std::string& cache) : tris_(std::move(tris)
     tn_buffer_free(nodes);
 }
 
-void KDTree::create(const uint64_t* nodes, size_t nn, const float* box) {
+KDTree::KDTree(Triangles tris, uint32_t scene_flags) : tris_(std::move(tris)) {
+    create(nullptr, 0, nullptr, scene_flags);
+}
+
+void KDTree::create(const uint64_t* nodes, size_t nn, const float* box, uint32_t flags) {
     std::vector<float> v, n, a, e;
     flatten(tris_, v, n, a, e);
     if (nodes)
         check(tn_scene_create_with_tree(v.data(), n.data(), a.data(), e.data(), (uint32_t)tris_.size(),
                                         nodes, nn, box, &scene_));
     else
-        check(tn_scene_create(v.data(), n.data(), a.data(), e.data(), (uint32_t)tris_.size(), 0, &scene_));
+        check(tn_scene_create(v.data(), n.data(), a.data(), e.data(), (uint32_t)tris_.size(), flags, &scene_));
     check(tn_scene_get_info(scene_, &info_));
 }
 
@@ -263,6 +267,73 @@ Scene load_tns(const std::string& path) {
         std::memcpy(&t.emissive, &e[4 * i], 16);
     }
     return sc;
+}
+
+Guide::Guide(const KDTree& tree, uint64_t seed, float lr, uint32_t batch, uint32_t epochs, float mis_guided) {
+    const float* box = tree.box();
+    check(tn_guide_create(box, box + 3, seed, &g_));
+    const int rc = tn_guide_set_options(g_, lr, batch, mis_guided, epochs);
+    if (rc < 0) {
+        const std::string msg = tn_last_error();
+        tn_guide_destroy(g_);
+        g_ = nullptr;
+        throw Error(rc, "turner_amd: " + msg);
+    }
+}
+
+Guide::~Guide() {
+    if (g_) tn_guide_destroy(g_);
+}
+
+RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vector<Light>& lights,
+                           const TracerConfig& conf, size_t height, Guide& guide, const GuidedSchedule& sch) {
+    if (conf.num_monte_carlo_samples != 1)
+        throw Error(TN_ERR_INVALID, "guided rendering trains on one path per sample (-m 1)");
+    RenderResult res;
+    const size_t np = conf.width * height;
+    res.linear.resize(np);
+    res.image.resize(np);
+    tn_camera c{};
+    std::memcpy(c.pos, &cam.position, 12);
+    std::memcpy(c.trafo, cam.trafo, sizeof c.trafo);
+    c.delta_x = cam.delta_x;
+    c.delta_y = cam.delta_y;
+    const auto ls = lights_of(lights);
+    DeviceBuffer acc(np * 16 * 3);  // image sums, training scratch, tone-mapped
+    float* img = acc.as<float>();
+    float* scratch = img + np * 4;
+    float* shown = img + np * 8;
+    if (hipMemset(img, 0, np * 16 * 2) != hipSuccess) throw Error(TN_ERR_HIP, "hipMemset failed");
+    tn_render_params p = params_of(conf, conf.width, height);
+    uint32_t offset = 0;
+    auto run = [&](int spp, float* into, uint32_t train) {
+        p.pixel_samples = spp;
+        p.sample_offset = offset;
+        tn_render_stats st{};
+        check(tn_render_guided(tree.handle(), &c, ls.data(), (uint32_t)ls.size(), &p, guide.handle(), train,
+                               into, &st, nullptr));
+        offset += (uint32_t)spp;
+        res.stats.rays += st.rays;
+        res.stats.primary_rays += st.primary_rays;
+        res.stats.shadow_rays += st.shadow_rays;
+        res.stats.ms += st.ms;
+    };
+    for (int k = 0; k < sch.train_passes; ++k) run(sch.train_spp, scratch, sch.train_samples);
+    int left = conf.num_pixel_samples;
+    while (left > 0) {
+        const int n = std::min(left, std::max(1, sch.pass_spp));
+        run(n, img, 0);
+        left -= n;
+    }
+    std::vector<float> tmp(np * 4);
+    to_host(tmp.data(), img, np * 16);
+    const float inv = 1.0f / (float)conf.num_pixel_samples;
+    for (size_t k = 0; k < np; ++k)
+        res.linear[k] = Color{tmp[4 * k] * inv, tmp[4 * k + 1] * inv, tmp[4 * k + 2] * inv, tmp[4 * k + 3] * inv};
+    check(tn_tonemap(img, shown, (uint32_t)np, inv, conf.exposure, conf.inverse_gamma,
+                     conf.gamma_correction_enabled ? 1 : 0, nullptr));
+    to_host(res.image.data(), shown, np * 16);
+    return res;
 }
 
 }  // namespace turner_amd

@@ -72,6 +72,8 @@ public:
     // Uses a tree from the cache file when it belongs to these triangles,
     // builds (and caches) it otherwise.
     KDTree(Triangles tris, const std::string& cache_path);
+    // Scene flags TN_SCENE_* (e.g. TN_SCENE_EMPTY_LEAVES: the rendering tree).
+    KDTree(Triangles tris, uint32_t scene_flags);
     KDTree(const KDTree&) = delete;
     KDTree& operator=(const KDTree&) = delete;
     KDTree(KDTree&& o) noexcept;
@@ -89,7 +91,7 @@ public:
     double build_ms() const { return info_.build_ms; }
 
 private:
-    void create(const uint64_t* nodes, size_t n, const float* box);
+    void create(const uint64_t* nodes, size_t n, const float* box, uint32_t flags = 0);
     Triangles tris_;
     tn_scene* scene_ = nullptr;
     tn_scene_info info_{};
@@ -149,6 +151,37 @@ struct RenderResult {
 // The pixel loop of main.cpp:187-226.
 RenderResult render(const KDTree& tree, const Camera& cam, const std::vector<Light>& lights,
                     const TracerConfig& conf, size_t height);
+
+// Learned guiding proposal (include/turner_amd.h, tn_guide_*): no reference
+// counterpart.  Owns a tn_guide whose position encoding spans the tree's box.
+class Guide {
+public:
+    explicit Guide(const KDTree& tree, uint64_t seed = 1, float lr = 1e-3f, uint32_t batch = 16384,
+                   uint32_t epochs = 4, float mis_guided = 0.5f);
+    Guide(const Guide&) = delete;
+    Guide& operator=(const Guide&) = delete;
+    ~Guide();
+    tn_guide* handle() const { return g_; }
+
+private:
+    tn_guide* g_ = nullptr;
+};
+
+// Progressive guided rendering: train_passes passes of train_spp samples
+// per pixel train the guide online (each on a uniform subset of
+// train_samples radiance records) and stay out of the image; then passes of
+// pass_spp samples render conf.num_pixel_samples samples per pixel with the
+// guide (one-sample MIS with the cosine lobe, unbiased).  Needs
+// num_monte_carlo_samples == 1.
+struct GuidedSchedule {
+    int train_passes = 16;
+    int train_spp = 2;
+    uint32_t train_samples = 1u << 18;
+    int pass_spp = 64;
+};
+RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vector<Light>& lights,
+                           const TracerConfig& conf, size_t height, Guide& guide,
+                           const GuidedSchedule& schedule = GuidedSchedule());
 
 // PPM P3 as lib/raster.h:79-100 writes it.
 std::string to_ppm(const std::vector<Color>& image, size_t width, size_t height);
