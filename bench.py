@@ -438,7 +438,9 @@ def render_throughput(args):
     st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc, sample_offset=1)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
+    cornell = cornell_gpu_vs_cpu(args)
     return {
+        "cornell_320": cornell,
         "workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
                      "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
                      f"{args.render_spp} pixel samples timed, rendering kd-tree"),
@@ -446,6 +448,48 @@ def render_throughput(args):
         "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
         "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 3),
     }
+
+
+def cornell_gpu_vs_cpu(args):
+    """The reference's own CPU-runnable case: the 36-triangle Cornell box,
+    320x320, --max-depth 3, --monte-carlo-samples 1, --pixel-samples 8, the
+    reference estimator (uniform hemisphere, point light) with the reference's
+    pixel jitter; GPU render vs the oracle's restatement of trace() on the
+    host cores (the full render; reference rays/s = closest-hit segments per
+    second as lib/output.h:101-112 prints)."""
+    import torch
+
+    import oracle as O
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_RENDER_REF_JITTER
+    sc = scenes.cornell_box()
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    conf = T.TracerConfig(width=320, max_recursion_depth=3, num_pixel_samples=8, num_monte_carlo_samples=1,
+                          flags=TN_RENDER_REF_JITTER)
+    acc = torch.zeros((320, 320, 4), dtype=torch.float32, device="cuda")
+    T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)  # warm-up
+    best = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        best = sec if best is None else min(best, sec)
+    out = {"gpu_seconds": round(best, 4), "gpu_rays_per_s": round(st["rays"] / best, 1),
+           "gpu_all_query_per_s": round((st["rays"] + st["shadow_rays"]) / best, 1)}
+    if not args.no_cpu_baseline:
+        ncores = min(os.cpu_count() or 1, 16)
+        ot = O.Tree(sc["verts"])
+        t0 = time.perf_counter()
+        _, _, nr, ns = O.render(sc, ot, 320, 320, 3, 1, 8, flags=O.FLAG_REF_RNG, nthreads=ncores)
+        csec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"rays_per_s": round(nr / csec, 1), "all_query_per_s": round((nr + ns) / csec, 1),
+                               "cores": ncores, "kind": "port",
+                               "sample": f"the full 320x320x8 render ({csec:.2f} s), oracle restatement of trace()"}
+        out["gpu_over_cpu"] = round(out["gpu_rays_per_s"] / out["cpu_baseline"]["rays_per_s"], 1)
+    return out
 
 
 def guided_vs_plain(args):
