@@ -431,3 +431,27 @@ def test_trained_guide_lowers_error(gpu):
     print(f"[guiding gain] relMSE at {spp} spp: guided {eg:.5f}, cosine {ep:.5f} ({ep / eg:.2f}x)", flush=True)
     assert eg < 0.8 * ep
     g.close()
+
+
+def test_guide_errors(gpu):
+    """Invalid options and calls fail loudly with TurnerError."""
+    torch = _torch()
+    from turner_amd.guide import Guide, render_guided
+    g = Guide(BMIN, BMAX, seed=1)
+    for kw in ({"batch": 100}, {"batch": 0}, {"lr": 0.0}, {"mis_guided": 1.5}, {"epochs": 0}):
+        with pytest.raises(T.TurnerError):
+            g.set_options(**{**{"lr": 1e-3, "batch": 16384, "mis_guided": 0.5, "epochs": 1}, **kw})
+    with pytest.raises(T.TurnerError):
+        g.set_params(np.full(OG.n_params(), np.nan, np.float32))
+    empty = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(T.TurnerError):
+        g.loss_grad(empty, empty, empty, torch.zeros(0, device="cuda"))
+    sc = scenes.cornell_box()
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    conf = T.TracerConfig(width=8, max_recursion_depth=2, num_pixel_samples=1, num_monte_carlo_samples=2)
+    acc = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(T.TurnerError):  # training records need one path per sample
+        render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1024)
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc)  # inference-only guiding works with m > 1
+    assert torch.isfinite(acc).all()
+    g.close()
