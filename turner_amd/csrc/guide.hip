@@ -112,16 +112,20 @@ __global__ __launch_bounds__(64) void k_guide_eval(GuideView G, const float* __r
 
 // MLP outputs for n points with persistent waves: each wave (block) loads
 // every BF16 weight fragment into registers once and then runs groups of 64
-// points, so weights are not re-read per group.
+// points, so weights are not re-read per group; the products use gfx950's
+// K = 32 BF16 MFMA (half the instructions of the K = 16 form).
 __global__ __launch_bounds__(64) void k_guide_forward(GuideView G, const float* __restrict__ p3,
                                                       const float* __restrict__ n3, uint32_t n,
                                                       float* __restrict__ out) {
     __shared__ __align__(16) uint16_t lds[64 * (kGStrideIn + kGStrideHid)];
-    g_s4 w1[4][5], w2[4][4], w3[4][4], w4[3][4];
-    load_frags<5, 4>(G.wbf + kGW1, w1);
-    load_frags<4, 4>(G.wbf + kGW2, w2);
-    load_frags<4, 4>(G.wbf + kGW3, w3);
-    load_frags<4, 3>(G.wbf + kGW4, w4);
+    // K = 80 = 2 x 32 + 16 for layer 1, 64 = 2 x 32 for the others
+    Frags32<2, true, 4> w1;
+    Frags32<2, false, 4> w2, w3;
+    Frags32<2, false, 3> w4;
+    load_frags32(G.wbf + kGW1, w1);
+    load_frags32(G.wbf + kGW2, w2);
+    load_frags32(G.wbf + kGW3, w3);
+    load_frags32(G.wbf + kGW4, w4);
     const uint32_t l = threadIdx.x;
     uint16_t* a = lds;
     uint16_t* b = lds + 64 * kGStrideIn;
@@ -137,13 +141,13 @@ __global__ __launch_bounds__(64) void k_guide_forward(GuideView G, const float* 
         }
         guide_features(G, p, nn, act, a + l * kGStrideIn);
         __syncthreads();
-        mfma_wx_reg<5, 4>(w1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
+        mfma_wx32(w1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
         __syncthreads();
-        mfma_wx_reg<4, 4>(w2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
+        mfma_wx32(w2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
         __syncthreads();
-        mfma_wx_reg<4, 4>(w3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
+        mfma_wx32(w3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
         __syncthreads();
-        mfma_wx_reg<4, 3>(w4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, o});
+        mfma_wx32(w4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, o});
         __syncthreads();
         const uint32_t rows = min(64u, n - base);
         for (uint32_t k = l; k < rows * kGOut; k += 64) out[(size_t)base * kGOut + k] = o[k];

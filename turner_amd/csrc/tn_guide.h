@@ -217,6 +217,60 @@ __device__ __forceinline__ void load_frags(const uint16_t* __restrict__ W, g_s4 
             Wf[mt][kt] = *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * (16 * IN_TILES) + 16 * kt + 4 * grp);
 }
 
+typedef __attribute__((ext_vector_type(8))) __bf16 g_bf8;
+
+// K = 32 tiles with gfx950's mfma_f32_16x16x32_bf16: A lane holds row
+// 16mt + l%16, columns 32kt + 8(l/16) .. +7; B lane holds sample 16nt + l%16,
+// features 32kt + 8(l/16) .. +7; the accumulator layout is the 16x16x16 one.
+// KT32 tiles of 32 then, if TAIL, one 16x16x16 tile for columns 32 KT32 .. +15.
+template <int KT32, bool TAIL, int OUT_TILES>
+struct Frags32 {
+    g_bf8 a[OUT_TILES][KT32];
+    g_s4 t[OUT_TILES][TAIL ? 1 : 1];
+};
+
+template <int KT32, bool TAIL, int OUT_TILES>
+__device__ __forceinline__ void load_frags32(const uint16_t* __restrict__ W, Frags32<KT32, TAIL, OUT_TILES>& F) {
+    constexpr int kCols = 32 * KT32 + (TAIL ? 16 : 0);
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int mt = 0; mt < OUT_TILES; ++mt) {
+#pragma unroll
+        for (int kt = 0; kt < KT32; ++kt)
+            F.a[mt][kt] = *reinterpret_cast<const g_bf8*>(W + (16 * mt + row) * kCols + 32 * kt + 8 * grp);
+        if (TAIL) F.t[mt][0] = *reinterpret_cast<const g_s4*>(W + (16 * mt + row) * kCols + 32 * KT32 + 4 * grp);
+    }
+}
+
+template <int KT32, bool TAIL, int OUT_TILES, class Epi>
+__device__ __forceinline__ void mfma_wx32(const Frags32<KT32, TAIL, OUT_TILES>& F, const uint16_t* in,
+                                          int in_stride, Epi epi) {
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const uint16_t* x = in + (16 * nt + row) * in_stride;
+        g_bf8 b[KT32];
+#pragma unroll
+        for (int kt = 0; kt < KT32; ++kt) b[kt] = *reinterpret_cast<const g_bf8*>(x + 32 * kt + 8 * grp);
+        g_s4 bt = {0, 0, 0, 0};
+        if (TAIL) bt = *reinterpret_cast<const g_s4*>(x + 32 * KT32 + 4 * grp);
+#pragma unroll
+        for (int mt = 0; mt < OUT_TILES; ++mt) {
+            g_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < KT32; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.a[mt][kt], b[kt], acc, 0, 0, 0);
+            if (TAIL) {  // the K = 16 tail accumulates separately: no mixed-shape accumulation chain
+                g_f4 t = {0.f, 0.f, 0.f, 0.f};
+                t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.t[mt][0], bt, t, 0, 0, 0);
+                acc += t;
+            }
+            epi(16 * mt + 4 * grp, 16 * nt + row, acc);
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
 }
