@@ -526,10 +526,10 @@ def guided_vs_plain(args):
     T.render_accumulate(tree, sc["camera"], sc["lights"], tiny, scratch)
     render_guided(tree, sc["camera"], sc["lights"], tiny, g, scratch, train_samples=1 << 14)
     g.close()
-    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=64)
+    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=32)
     g = Guide.for_tree(tree, seed=2, epochs=4)
-    guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12, ref_var=ref_var, spp_per_pass=64,
-                            train_passes=16, train_spp=2, train_samples=1 << 18, keep_training_passes=False)
+    guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12, ref_var=ref_var, spp_per_pass=32,
+                            train_passes=8, train_spp=1, train_samples=1 << 18, keep_training_passes=False)
     ginfo = g.info()
     # MFMA rate of the guide kernels (HIP events on the launch stream)
     dev = torch.device("cuda:0")
@@ -565,10 +565,11 @@ def guided_vs_plain(args):
                      f"area-light NEE, rendering kd-tree, relMSE target 0.01 vs a {args.guided_ref_spp}-spp "
                      f"plain reference ({ref_s:.1f} s; its own per-pixel variance, relMSE "
                      f"{ref_noise:.4f}, is subtracted)"),
-        "guided_schedule": ("16 training passes of 2 spp, each training on a uniform 2^18-sample subset of "
-                            "its radiance records for 4 epochs (Adam lr 1e-3, batch 16384), then 64-spp "
+        "guided_schedule": ("8 training passes of 1 spp, each training on a uniform 2^18-sample subset of "
+                            "its radiance records for 4 epochs (Adam lr 1e-3, batch 16384), then 32-spp "
                             "passes; training passes are timed but left out of the image; one-sample MIS "
-                            "0.5 guide / 0.5 cosine"),
+                            "0.5 guide / 0.5 cosine; both tracers add 32-spp passes and sample emitters "
+                            "by solid angle"),
         "plain": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items() if k != "history"},
         "guided": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in guided.items() if k != "history"},
         "guided_time_over_plain": None if ratio is None else round(ratio, 4),

@@ -170,7 +170,10 @@ typedef struct {
 
 /* Render flags */
 #define TN_RENDER_COSINE 1u      /* cosine-weighted hemisphere (default: uniform, as the reference) */
-#define TN_RENDER_AREA_LIGHTS 2u /* emissive triangles: NEE + camera-visible emission */
+/* emissive triangles: camera-visible emission + one next-event sample per hit: an emitter by solid
+ * angle (scenes with <= 4 emitter triangles) or by area, then a direction uniform in its solid angle
+ * (Arvo 1995), or a point uniform by area when it subtends < 1e-3 sr */
+#define TN_RENDER_AREA_LIGHTS 2u
 #define TN_RENDER_REF_JITTER 4u  /* pixel jitter of main.cpp:201-206 (xorshift64* seeded 42 per row) */
 #define TN_RENDER_GUIDED 8u      /* learned guiding proposal (needs a tn_guide) */
 #define TN_RENDER_EXHAUSTIVE 16u /* reference traversal order without early exit */
@@ -345,6 +348,17 @@ int tn_batch_render(tn_batch* batch, const tn_batch_job* jobs, const tn_render_p
                     float* accum, tn_render_stats* stats, void* stream);
 /* The launch shape the autotuner chose for the last TN_RENDER_AUTOTUNE render. */
 int tn_batch_get_tuning(const tn_batch* batch, tn_batch_tuning* out);
+
+/*
+ * Test hook (no reference counterpart), host only: the shade kernel's
+ * solid-angle sampler of a triangle seen from a point, on the CPU.  p3: the
+ * point; tri9: the vertices; u2: n pairs (u1, u2) in [0, 1); writes n unit
+ * directions to dir3 and the sampler's solid angle (sr) to *omega and the
+ * triple-product solid angle used to pick among emitters to *omega_vos.
+ * TN_ERR_INVALID for a degenerate projection.
+ */
+int tn_sample_triangle_host(const float* p3, const float* tri9, const float* u2, uint32_t n, float* dir3,
+                            float* omega, float* omega_vos);
 
 const char* tn_last_error(void);
 /* Device name and compute-unit count of the current HIP device. */

@@ -136,6 +136,7 @@ SIGNATURES = {
     "tn_batch_destroy": (C.c_int, [_P]),
     "tn_batch_render": (C.c_int, [_P, _P, C.POINTER(RenderParams), _P, C.POINTER(RenderStats), _P]),
     "tn_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "tn_sample_triangle_host": (C.c_int, [_P, _P, _P, _U32, _P, _FP, _FP]),
     "tn_last_error": (C.c_char_p, []),
     "tn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
 }

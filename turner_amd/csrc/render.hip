@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "tn_guide.h"
+#include "tn_sample.h"
 #include "tn_scene.h"
 
 namespace tn {
@@ -57,6 +58,8 @@ namespace {
 constexpr int kRBlock = 256;
 constexpr uint32_t kMaxLevels = 64;
 constexpr uint64_t kSceneSeedStep = 0x9E3779B97F4A7C15ull;
+// scenes with at most this many emitter triangles pick one by solid angle
+constexpr uint32_t kSolidAngleSelect = 4;
 
 struct Queues {
     float4* o[2];   // o.xyz, pixel (bits)
@@ -450,37 +453,93 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                     if (level == 0 && (le.x > 0 || le.y > 0 || le.z > 0) &&
                         g.x * qd.x + g.y * qd.y + g.z * qd.z < 0)
                         accumulate(A.accum, pixel, mul4(w, le));
-                    // one emitter by area, uniform point on it
                     const float u0 = rng_uniform(rng), u1 = rng_uniform(rng), u2 = rng_uniform(rng);
-                    const float x = u0 * S.emit_area;
-                    const float* cdf = A.emit_cdf + S.emit_begin;
-                    uint32_t lo = 0, hi = S.nemit - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (cdf[mid] <= x) lo = mid + 1;
-                        else hi = mid;
+                    // one emitter k with probability pk: by solid angle among
+                    // the front-facing ones of a scene with few emitters, else
+                    // by area
+                    uint32_t k = 0;
+                    float pk = 0.f;
+                    if (S.nemit <= kSolidAngleSelect) {
+                        float wk[kSolidAngleSelect];
+                        float wsum = 0.f;
+#pragma unroll
+                        for (uint32_t j = 0; j < kSolidAngleSelect; ++j) {
+                            wk[j] = 0.f;
+                            if (j < S.nemit) {
+                                const uint32_t kj = A.emit_ids[S.emit_begin + j];
+                                const float4 r0 = A.trirec[4 * (size_t)kj], rn = A.trirec[4 * (size_t)kj + 1],
+                                             ru = A.trirec[4 * (size_t)kj + 2], rv = A.trirec[4 * (size_t)kj + 3];
+                                const float3 a = make_float3(r0.x - p.x, r0.y - p.y, r0.z - p.z);
+                                if (-(a.x * rn.x + a.y * rn.y + a.z * rn.z) > 0.f)  // p in front of the emitter
+                                    wk[j] = tri_solid_angle(a, v3axpy(1.f, make_float3(ru.x, ru.y, ru.z), a),
+                                                            v3axpy(1.f, make_float3(rv.x, rv.y, rv.z), a));
+                            }
+                            wsum += wk[j];
+                        }
+                        const float x = u0 * wsum;
+                        float run = 0.f;
+                        bool picked = false;
+#pragma unroll
+                        for (uint32_t j = 0; j < kSolidAngleSelect; ++j) {
+                            run += wk[j];
+                            if (!picked && wk[j] > 0.f && (x < run || run >= wsum)) {
+                                picked = true;
+                                k = A.emit_ids[S.emit_begin + j];
+                                pk = wk[j] / wsum;
+                            }
+                        }
+                    } else {
+                        const float x = u0 * S.emit_area;
+                        const float* cdf = A.emit_cdf + S.emit_begin;
+                        uint32_t lo = 0, hi = S.nemit - 1;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (cdf[mid] <= x) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        k = A.emit_ids[S.emit_begin + lo];
+                        const float cprev = lo ? cdf[lo - 1] : 0.f;
+                        pk = (cdf[lo] - cprev) / S.emit_area;
                     }
-                    const uint32_t k = A.emit_ids[S.emit_begin + lo];
-                    const float4 e0 = A.trirec[4 * (size_t)k], en = A.trirec[4 * (size_t)k + 1],
-                                 eu = A.trirec[4 * (size_t)k + 2], ev = A.trirec[4 * (size_t)k + 3];
-                    const float su = sqrtf(u1);
-                    const float bu = su * (1.0f - u2), bv = su * u2;
-                    const float3 q = make_float3(e0.x + bu * eu.x + bv * ev.x, e0.y + bu * eu.y + bv * ev.y,
-                                                 e0.z + bu * eu.z + bv * ev.z);
-                    const float3 dl = make_float3(q.x - p.x, q.y - p.y, q.z - p.z);
-                    const float d2 = dl.x * dl.x + dl.y * dl.y + dl.z * dl.z;
-                    const float id_ = 1.0f / sqrtf(d2);
-                    adir = make_float3(dl.x * id_, dl.y * id_, dl.z * id_);
-                    const float cosx = adir.x * nn.x + adir.y * nn.y + adir.z * nn.z;
-                    const float cosl = -(adir.x * en.x + adir.y * en.y + adir.z * en.z);
-                    if (cosx > 0 && cosl > 0) {
-                        const float3 q2 = make_float3(q.x - p2.x, q.y - p2.y, q.z - p2.z);
-                        adist = sqrtf(q2.x * q2.x + q2.y * q2.y + q2.z * q2.z) * (1.0f - 1e-4f);
-                        const float4 lek = A.emission[k];
-                        const float gterm = cosx * cosl / d2 * S.emit_area;
-                        acontrib = scale4(inv_pi, mul4(wr, scale4(gterm, lek)));
-                        if (records) alocal = scale4(inv_pi, mul4(rho, scale4(gterm, lek)));
-                        has_area = true;
+                    if (pk > 0.f) {
+                        const float4 e0 = A.trirec[4 * (size_t)k], en = A.trirec[4 * (size_t)k + 1],
+                                     eu = A.trirec[4 * (size_t)k + 2], ev = A.trirec[4 * (size_t)k + 3];
+                        const float3 a = make_float3(e0.x - p.x, e0.y - p.y, e0.z - p.z);
+                        const float3 n3 = make_float3(en.x, en.y, en.z);
+                        const float3 eu3 = make_float3(eu.x, eu.y, eu.z), ev3 = make_float3(ev.x, ev.y, ev.z);
+                        // direction and inverse density per steradian
+                        float inv_pdf = 0.f;
+                        float3 q;
+                        SphTri st;
+                        if (sph_tri_setup(a, v3axpy(1.f, eu3, a), v3axpy(1.f, ev3, a), st) &&
+                            st.omega >= kMinSolidAngle) {
+                            adir = sph_tri_sample(st, u1, u2);
+                            const float dn = v3dot(adir, n3);  // = -cos_l
+                            const float tq = dn < 0.f ? v3dot(a, n3) / dn : 0.f;
+                            q = v3axpy(tq, adir, p);
+                            inv_pdf = dn < 0.f ? st.omega : 0.f;
+                        } else {  // uniform point by area: density d^2 / (cos_l area)
+                            const float su = sqrtf(u1);
+                            const float bu = su * (1.0f - u2), bv = su * u2;
+                            q = make_float3(e0.x + bu * eu.x + bv * ev.x, e0.y + bu * eu.y + bv * ev.y,
+                                            e0.z + bu * eu.z + bv * ev.z);
+                            const float3 dl = v3sub(q, p);
+                            const float d2 = v3dot(dl, dl);
+                            adir = v3scale(1.0f / sqrtf(d2), dl);
+                            const float cl = -v3dot(adir, n3);
+                            const float3 cr = v3cross(eu3, ev3);
+                            inv_pdf = cl > 0.f ? cl * 0.5f * sqrtf(v3dot(cr, cr)) / d2 : 0.f;
+                        }
+                        const float cosx = v3dot(adir, nn);
+                        if (cosx > 0 && inv_pdf > 0) {
+                            const float3 q2 = v3sub(q, p2);
+                            adist = sqrtf(v3dot(q2, q2)) * (1.0f - 1e-4f);
+                            const float4 lek = A.emission[k];
+                            const float gterm = cosx * inv_pdf / pk;
+                            acontrib = scale4(inv_pi, mul4(wr, scale4(gterm, lek)));
+                            if (records) alocal = scale4(inv_pi, mul4(rho, scale4(gterm, lek)));
+                            has_area = true;
+                        }
                     }
                 }
             }
@@ -1178,6 +1237,27 @@ int autotune(tn_batch* b, const std::vector<SceneParams>& table, uint64_t total,
 using namespace tn;
 
 extern "C" {
+
+int tn_sample_triangle_host(const float* p3, const float* tri9, const float* u2, uint32_t n, float* dir3,
+                            float* omega, float* omega_vos) {
+    if (!p3 || !tri9 || (n && (!u2 || !dir3)) || !omega || !omega_vos)
+        return set_error(TN_ERR_INVALID, "NULL argument");
+    const float3 p = make_float3(p3[0], p3[1], p3[2]);
+    const float3 a = v3sub(make_float3(tri9[0], tri9[1], tri9[2]), p),
+                 b = v3sub(make_float3(tri9[3], tri9[4], tri9[5]), p),
+                 c = v3sub(make_float3(tri9[6], tri9[7], tri9[8]), p);
+    SphTri st;
+    if (!sph_tri_setup(a, b, c, st)) return set_error(TN_ERR_INVALID, "degenerate spherical triangle");
+    *omega = st.omega;
+    *omega_vos = tri_solid_angle(a, b, c);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float3 d = sph_tri_sample(st, u2[2 * i], u2[2 * i + 1]);
+        dir3[3 * i] = d.x;
+        dir3[3 * i + 1] = d.y;
+        dir3[3 * i + 2] = d.z;
+    }
+    return TN_OK;
+}
 
 int tn_render_accumulate(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32_t nl,
                          const tn_render_params* P, float* accum, tn_render_stats* st, void* stream) {
