@@ -82,6 +82,10 @@ int tn_scene_create_with_tree(const float* verts9, const float* normals9,
                               const float* albedo4, const float* emission4,
                               uint32_t num_triangles, const uint64_t* flat_nodes,
                               size_t num_nodes, const float* box6, tn_scene** out);
+/* Reflective colour (RGBA, 4 per triangle) and reflectivity (1 per triangle)
+ * of Triangle (lib/triangle.h:23-33, main.cpp:40-47), read by the raytracer
+ * (raytracer.cpp:37-46).  Without this call every reflectivity is 0. */
+int tn_scene_set_reflection(tn_scene* scene, const float* reflective4, const float* reflectivity);
 int tn_scene_destroy(tn_scene* scene);
 int tn_scene_get_info(const tn_scene* scene, tn_scene_info* info);
 /* Copies the reference-format FlatNode array (cap = capacity in nodes). */
@@ -193,8 +197,17 @@ typedef struct {
     int32_t gamma_enabled; /* !--no-gamma-correction */
     uint32_t chunk_paths;  /* primary samples per wavefront chunk (0: automatic) */
     uint32_t sample_offset;/* first pixel-sample index (progressive rendering) */
+    uint32_t renderer;     /* TN_RENDERER_*: which of the reference's trace() plugins */
+    float max_visibility;  /* raycaster --max-visibility (reference default 2) */
+    float shadow_intensity;/* raytracer --shadow (reference default 0.5) */
     uint32_t reserved;
 } tn_render_params;
+
+/* The reference's trace() plugins (trace.h:23-25; CMakeLists.txt:62-67 links
+ * one per binary).  All three run on the same traversal and queues. */
+#define TN_RENDERER_PATHTRACER 0u /* pathtracer.cpp:14-102 */
+#define TN_RENDERER_RAYCASTER 1u  /* raycaster.cpp:7-24: diffuse colour, alpha fading with distance */
+#define TN_RENDERER_RAYTRACER 2u  /* raytracer.cpp:6-67: Lambert + mirror reflection + shadow darkening */
 
 typedef struct {
     uint64_t rays;        /* closest-hit segments, the reference's num_rays (pathtracer.cpp:21) */

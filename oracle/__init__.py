@@ -31,13 +31,15 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_int32),
                 ("mc_samples", C.c_int32), ("pixel_samples", C.c_int32),
                 ("flags", C.c_uint32), ("seed", C.c_uint64), ("bg", C.c_float * 4),
-                ("nthreads", C.c_int32), ("row_step", C.c_uint32), ("row_offset", C.c_uint32)]
+                ("nthreads", C.c_int32), ("row_step", C.c_uint32), ("row_offset", C.c_uint32),
+                ("renderer", C.c_uint32), ("max_visibility", C.c_float), ("shadow_intensity", C.c_float)]
 
 
 class Scene(C.Structure):
     _fields_ = [("pre16", C.c_void_p), ("verts9", C.c_void_p), ("normals9", C.c_void_p),
                 ("albedo4", C.c_void_p), ("emission4", C.c_void_p), ("ntris", C.c_uint32),
-                ("nodes", C.c_void_p), ("box", C.c_float * 6)]
+                ("nodes", C.c_void_p), ("box", C.c_float * 6), ("reflective4", C.c_void_p),
+                ("reflectivity", C.c_void_p)]
 
 
 _lib = None
@@ -203,9 +205,16 @@ def raster2cam(cam, px, py, w, h):
     return out
 
 
+RENDERER_PATHTRACER = 0
+RENDERER_RAYCASTER = 1
+RENDERER_RAYTRACER = 2
+
+
 def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(0, 0, 0, 1),
-           nthreads=8, row_step=1, row_offset=0, want_sq=False):
-    """Oracle render: returns (mean RGBA (h, w, 4), sum of squares or None, rays, shadow rays)."""
+           nthreads=8, row_step=1, row_offset=0, want_sq=False, renderer=RENDERER_PATHTRACER,
+           max_visibility=2.0, shadow_intensity=0.5):
+    """Oracle render: returns (mean RGBA (h, w, 4), sum of squares or None, rays, shadow rays).
+    scene may hold "reflective" (N, 4) and "reflectivity" (N,) for the raytracer."""
     keep = []
 
     def arr(a, cols):
@@ -222,6 +231,9 @@ def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(
     sc.ntris = len(tree.pre)
     sc.nodes = tree.nodes.ctypes.data
     sc.box[:] = [float(x) for x in tree.box]
+    if "reflectivity" in scene:
+        sc.reflective4 = arr(scene["reflective"], 4).ctypes.data
+        sc.reflectivity = arr(scene["reflectivity"], 1).ctypes.data
     lights = scene.get("lights", [])
     la = (Light * max(1, len(lights)))()
     for k, (p, col) in enumerate(lights):
@@ -233,6 +245,7 @@ def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(
     prm.flags, prm.seed = flags, seed
     prm.bg[:] = [float(x) for x in bg]
     prm.nthreads, prm.row_step, prm.row_offset = nthreads, row_step, row_offset
+    prm.renderer, prm.max_visibility, prm.shadow_intensity = renderer, max_visibility, shadow_intensity
     out = np.zeros((height, width, 4), np.float32)
     sq = np.zeros((height, width, 4), np.float32) if want_sq else None
     nr, ns = C.c_uint64(0), C.c_uint64(0)

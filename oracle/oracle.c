@@ -1208,6 +1208,69 @@ static col4 trace(tracer* T, v3 o, v3 d, int depth) {
     return cadd(res, emitted);
 }
 
+/* raycaster.cpp:7-24: the hit triangle's diffuse colour, alpha fading with
+ * the ray parameter (clamp of src/turner.h:28-36); num_rays counts hits. */
+static col4 raycast(tracer* T, v3 o, v3 d) {
+    const or_render_params* P = T->prm;
+    float dist, s, t;
+    uint32_t id = closest(T, o, d, &dist, &s, &t);
+    if (id == OR_INVALID_ID) return c4(P->bg[0], P->bg[1], P->bg[2], P->bg[3]);
+    T->nrays++;
+    const float* alb = T->sc->albedo4 + 4 * (size_t)id;
+    float a = 1.f - dist / P->max_visibility;
+    a = a < 0.f ? 0.f : (a > 1.f ? 1.f : a);
+    return c4(alb[0], alb[1], alb[2], a);
+}
+
+/* raytracer.cpp:6-67: Lambert term of the (first) point light
+ * (lib/lambertian.h:15-19), mirror reflection weighted by reflectivity and
+ * the reflective colour, and darkening by the shadow intensity when the
+ * closest hit toward the light from p2 is nearer than the light. */
+static col4 raytrace(tracer* T, v3 o, v3 d, int depth) {
+    const or_render_params* P = T->prm;
+    T->nrays++;
+    if (depth > P->max_depth) return c4(0, 0, 0, 0);
+    const or_light* L = &T->lights[0];
+    float dist, s, t;
+    uint32_t id = closest(T, o, d, &dist, &s, &t);
+    if (id == OR_INVALID_ID) return c4(P->bg[0], P->bg[1], P->bg[2], P->bg[3]);
+    const or_scene* sc = T->sc;
+    v3 p = add(o, scale(dist, d));
+    v3 lp = load3(L->pos);
+    v3 ldir = normalize(sub(lp, p));
+    v3 n = interp_normal(sc->normals9 + 9 * (size_t)id, 1.f - s - t, s, t);
+    const float* alb = sc->albedo4 + 4 * (size_t)id;
+    float ca = fmaxf(0.f, dot(ldir, n));
+    col4 direct = cmul(cscale(ca, c4(alb[0], alb[1], alb[2], alb[3])),
+                       c4(L->color[0], L->color[1], L->color[2], L->color[3]));
+    v3 p2 = add(p, scale(0.0001f, n));
+    col4 color = direct;
+    float refl = sc->reflectivity ? sc->reflectivity[id] : 0.f;
+    if (refl > 0) {
+        v3 rd = sub(d, scale(2.f * dot(n, d), n));
+        col4 rc = raytrace(T, p2, rd, depth + 1);
+        const float* R = sc->reflective4 + 4 * (size_t)id;
+        color = cadd(cscale(1.f - refl, direct), cmul(cscale(refl, c4(R[0], R[1], R[2], R[3])), rc));
+    }
+    v3 ld2 = normalize(sub(lp, p2));
+    float dl = length(sub(lp, p2));
+    float dn, ss, tt;
+    T->nshadow++;
+    uint32_t sh = closest(T, p2, ld2, &dn, &ss, &tt);
+    if (sh != OR_INVALID_ID && dn < dl) {
+        float si = P->shadow_intensity;
+        color = c4(color.r - color.r * si, color.g - color.g * si, color.b - color.b * si, color.a - color.a * si);
+    }
+    return color;
+}
+
+/* the renderer's trace() (trace.h:23-25) for a camera ray */
+static col4 trace_plugin(tracer* T, v3 o, v3 d) {
+    if (T->prm->renderer == OR_RENDERER_RAYCASTER) return raycast(T, o, d);
+    if (T->prm->renderer == OR_RENDERER_RAYTRACER) return raytrace(T, o, d, 0);
+    return trace(T, o, d, 0);
+}
+
 typedef struct {
     tracer proto;
     const or_camera* cam;
@@ -1239,7 +1302,7 @@ static void render_rows(render_job* J, int ref_single_stream) {
                 float dy = or_xorshift64s_float(&jit);
                 float dir[3];
                 or_raster2cam(J->cam, (float)x + dx, (float)y + dy, (float)w, (float)h, dir);
-                col4 L = trace(&T, load3(J->cam->pos), load3(dir), 0);
+                col4 L = trace_plugin(&T, load3(J->cam->pos), load3(dir));
                 acc = cadd(acc, L);
                 acc2 = cadd(acc2, cmul(L, L));
             }

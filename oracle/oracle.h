@@ -108,7 +108,14 @@ typedef struct {
     int32_t nthreads;
     uint32_t row_step; /* render rows y % row_step == row_offset only */
     uint32_t row_offset;
+    uint32_t renderer;      /* OR_RENDERER_* */
+    float max_visibility;   /* raycaster */
+    float shadow_intensity; /* raytracer */
 } or_render_params;
+
+#define OR_RENDERER_PATHTRACER 0u /* pathtracer.cpp:14-102 */
+#define OR_RENDERER_RAYCASTER 1u  /* raycaster.cpp:7-24 */
+#define OR_RENDERER_RAYTRACER 2u  /* raytracer.cpp:6-67 (needs one light) */
 
 #define OR_FLAG_COSINE 1u     /* cosine-weighted hemisphere sampling */
 #define OR_FLAG_AREA_LIGHTS 2u /* emissive triangles: NEE + camera-visible emission */
@@ -124,6 +131,8 @@ typedef struct {
     uint32_t ntris;
     const uint64_t* nodes;
     float box[6];
+    const float* reflective4;  /* raytracer: RGBA per triangle (NULL: no reflection) */
+    const float* reflectivity; /* raytracer: per triangle */
 } or_scene;
 
 /* Renders linear RGBA (sum over samples / p, before exposure) into out

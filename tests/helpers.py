@@ -97,3 +97,28 @@ def hard_rays(verts, n, rng):
         d[i] = 0
         d[i, ax[i]] = 1
     return o.astype(np.float32), d.astype(np.float32)
+
+
+def wall_scene(z=3.0, occluder=False, reflectivity=None, light=(0.5, 0.5, 1.0)):
+    """Closed-form scene for the raycaster and raytracer: a wall at z = `z`
+    facing a camera at the origin that looks along +z (every primary ray has
+    d.z = 1, so it hits the wall at ray parameter r = z exactly), one point
+    light in front of the wall, optionally a small occluder quad at z = 2
+    between light and wall, optionally the wall's reflectivity (reflective
+    colour (0.9, 0.8, 0.7, 1))."""
+    from turner_amd import scenes as S
+    wall = S._pack(S._quad([-20, -20, z], [-20, 20, z], [20, 20, z], [20, -20, z]), (0.6, 0.5, 0.4))
+    parts = [wall]
+    if occluder:
+        parts.append(S._pack(S._quad([0.3, 0.3, 2.0], [0.3, 0.9, 2.0], [0.9, 0.9, 2.0], [0.9, 0.3, 2.0]),
+                             (0.2, 0.7, 0.3)))
+    verts, normals, albedo, emission = S.concat(*parts)
+    sc = {"verts": verts, "normals": normals, "albedo": albedo, "emission": emission,
+          "camera": S.camera((0.0, 0.0, 0.0), 60.0),
+          "lights": [(light, (1.0, 0.9, 0.8, 1.0))]}
+    n = len(verts)
+    sc["reflective"] = np.tile(np.array([0.9, 0.8, 0.7, 1.0], np.float32), (n, 1))
+    sc["reflectivity"] = np.zeros(n, np.float32)
+    if reflectivity is not None:
+        sc["reflectivity"][:2] = reflectivity
+    return sc

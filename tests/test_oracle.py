@@ -271,3 +271,96 @@ def test_reference_cornell_ray_counts():
     ratio = nr / (80 * 80 * 8)
     print("segments per primary", ratio)
     assert 2.9 < ratio < 3.5
+
+
+# ------------------------------------------------------------ raycaster / raytracer
+def _plugin(sc, renderer, w=24, depth=3, bg=(0.1, 0.2, 0.3, 1.0), **kw):
+    t = O.Tree(sc["verts"])
+    img, _, nr, ns = O.render(sc, t, w, w, depth, 1, 1, flags=O.FLAG_REF_RNG, bg=bg, nthreads=1,
+                              renderer=renderer, **kw)
+    return img, nr, ns
+
+
+def _primary_dirs(cam, w):
+    """main.cpp:201-209 with one pixel sample: per row gen(42), (dx, dy) per pixel."""
+    u = O.xorshift64s_floats(42, 2 * w)
+    return np.array([[O.raster2cam(cam, x + u[2 * x], y + u[2 * x + 1], w, w) for x in range(w)]
+                     for y in range(w)], np.float64)
+
+
+def _lambert(p, n, albedo, light):
+    lp, lc = np.asarray(light[0], np.float64), np.asarray(light[1], np.float64)
+    l = lp - p
+    l /= np.linalg.norm(l, axis=-1, keepdims=True)
+    ca = np.maximum(0.0, (l * n).sum(-1))
+    return ca[..., None] * np.asarray(albedo, np.float64) * lc
+
+
+def test_raycaster_closed_form():
+    """raycaster.cpp:7-24: every ray hits the wall at r = 3 exactly, so each
+    pixel is the diffuse colour with alpha clamp(1 - 3 / max_visibility)."""
+    sc = H.wall_scene()
+    for mv in (5.0, 2.0, 3.0):
+        img, nr, ns = _plugin(sc, O.RENDERER_RAYCASTER, max_visibility=mv)
+        a = np.float32(1) - np.float32(3) / np.float32(mv)
+        a = min(max(a, np.float32(0)), np.float32(1))
+        assert np.array_equal(img[..., :3], np.broadcast_to(np.float32([0.6, 0.5, 0.4]), img[..., :3].shape))
+        assert np.all(img[..., 3] == a)
+        assert nr == 24 * 24 and ns == 0  # num_rays counts hits only
+
+
+def test_raycaster_background_not_counted():
+    sc = H.wall_scene(z=-3.0)  # behind the camera: every ray misses
+    img, nr, _ = _plugin(sc, O.RENDERER_RAYCASTER)
+    assert nr == 0 and np.all(img == np.float32([0.1, 0.2, 0.3, 1.0]))
+
+
+def test_raytracer_lambert_closed_form():
+    """raytracer.cpp:24-34: Lambert term of the point light (lib/lambertian.h:15-19), lit."""
+    sc = H.wall_scene()
+    img, nr, ns = _plugin(sc, O.RENDERER_RAYTRACER)
+    d = _primary_dirs(sc["camera"], 24)
+    p = 3.0 * d
+    want = _lambert(p, np.array([0, 0, -1.0]), (0.6, 0.5, 0.4, 1.0), sc["lights"][0])
+    assert np.allclose(img, want, rtol=2e-6, atol=1e-7)
+    assert nr == 24 * 24 and ns == 24 * 24
+
+
+def test_raytracer_shadow_closed_form():
+    """raytracer.cpp:55-64: a wall point whose segment to the light crosses the
+    occluder (at z = 2, halfway to the light) is darkened by c -= c * 0.5."""
+    sc = H.wall_scene(occluder=True)
+    img, _, _ = _plugin(sc, O.RENDERER_RAYTRACER, w=32, shadow_intensity=0.5)
+    d = _primary_dirs(sc["camera"], 32)
+    light = sc["lights"][0]
+    lo, hi, m = 0.3, 0.9, 1e-3
+    q = 2.0 * d[..., :2]  # primary ray at the occluder's plane
+    on_occ = np.all((q > lo + m) & (q < hi - m), -1)
+    off_occ = np.any((q < lo - m) | (q > hi + m), -1)
+    p = 3.0 * d
+    c = 0.5 * (p[..., :2] + np.asarray(light[0][:2]))  # segment to the light at z = 2
+    shadowed = off_occ & np.all((c > lo + m) & (c < hi - m), -1)
+    lit = off_occ & np.any((c < lo - m) | (c > hi + m), -1)
+    wall = _lambert(p, np.array([0, 0, -1.0]), (0.6, 0.5, 0.4, 1.0), light)
+    occ = _lambert(2.0 * d, np.array([0, 0, -1.0]), (0.2, 0.7, 0.3, 1.0), light)
+    assert shadowed.sum() > 20 and lit.sum() > 20 and on_occ.sum() > 20
+    assert np.allclose(img[lit], wall[lit], rtol=2e-6, atol=1e-7)
+    assert np.allclose(img[shadowed], 0.5 * wall[shadowed], rtol=2e-6, atol=1e-7)
+    assert np.allclose(img[on_occ], occ[on_occ], rtol=2e-6, atol=1e-7)
+
+
+def test_raytracer_reflection_closed_form():
+    """raytracer.cpp:42-52: the mirror ray d - 2 (n.d) n leaves the scene, so a
+    wall of reflectivity 0.5 shows 0.5 direct + 0.5 reflective * background;
+    at max_depth 0 the mirror call returns {} (and is still counted)."""
+    sc = H.wall_scene(reflectivity=0.5)
+    bg = (0.1, 0.2, 0.3, 1.0)
+    d = _primary_dirs(sc["camera"], 24)
+    direct = _lambert(3.0 * d, np.array([0, 0, -1.0]), (0.6, 0.5, 0.4, 1.0), sc["lights"][0])
+    img, nr, ns = _plugin(sc, O.RENDERER_RAYTRACER, depth=1, bg=bg)
+    want = 0.5 * direct + 0.5 * np.array([0.9, 0.8, 0.7, 1.0]) * np.array(bg)
+    assert np.allclose(img, want, rtol=2e-6, atol=1e-7)
+    assert nr == 2 * 24 * 24 and ns == 24 * 24  # the mirror rays miss before their shadow test
+    img0, nr0, _ = _plugin(sc, O.RENDERER_RAYTRACER, depth=0, bg=bg)
+    assert np.allclose(img0, 0.5 * direct, rtol=2e-6, atol=1e-7)
+    assert nr0 == 2 * 24 * 24

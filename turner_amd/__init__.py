@@ -19,11 +19,13 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import TN_INVALID_ID, TurnerError, check, lib, ptr
+from ._lib import (TN_INVALID_ID, TN_RENDERER_PATHTRACER, TN_RENDERER_RAYCASTER, TN_RENDERER_RAYTRACER,
+                   TurnerError, check, lib, ptr)
 
 __all__ = ["KDTree", "KDTreeIntersection", "TracerConfig", "render", "TurnerError",
            "TN_INVALID_ID", "device_info", "build_tree", "tree_cache_save", "tree_cache_load",
-           "write_ppm", "render_accumulate"]
+           "write_ppm", "render_accumulate", "TN_RENDERER_PATHTRACER", "TN_RENDERER_RAYCASTER",
+           "TN_RENDERER_RAYTRACER"]
 
 
 def build_tree(verts, serial=False):
@@ -129,6 +131,15 @@ class KDTree:
         self._info = _lib.SceneInfo()
         check(lib().tn_scene_get_info(self.handle, C.byref(self._info)))
 
+    def set_reflection(self, reflective, reflectivity):
+        """Reflective colour (N, 4) and reflectivity (N,) per triangle, read by
+        the raytracer (raytracer.cpp:37-46; lib/triangle.h:23-33)."""
+        self._rc = _f32(reflective, 4)
+        self._rf = np.ascontiguousarray(reflectivity, np.float32).reshape(-1)
+        if self._rc is None or len(self._rc) != len(self._v) or len(self._rf) != len(self._v):
+            raise ValueError("one reflective colour and one reflectivity per triangle")
+        check(lib().tn_scene_set_reflection(self.handle, ptr(self._rc), ptr(self._rf)))
+
     def close(self):
         if getattr(self, "handle", None):
             lib().tn_scene_destroy(self.handle)
@@ -231,6 +242,10 @@ class TracerConfig:
     flags: int = 0
     seed: int = 1
     chunk_paths: int = 0
+    # the other trace() plugins (TN_RENDERER_*; raycaster.h / raytracer.h defaults)
+    renderer: int = 0
+    max_visibility: float = 2.0
+    shadow_intensity: float = 0.5
 
     def height(self):
         return int(self.width / self.aspect)
@@ -239,6 +254,7 @@ class TracerConfig:
         assert 0 < self.aspect and 1 <= self.num_threads and 0 <= self.exposure
         assert 0 < self.max_recursion_depth and 1 <= self.num_pixel_samples
         assert 0 <= self.num_monte_carlo_samples
+        assert 0 <= self.max_visibility and 0 <= self.shadow_intensity <= 1
 
 
 def _camera(cam):
@@ -271,6 +287,9 @@ def _params(conf, width, height):
     p.inverse_gamma = conf.inverse_gamma
     p.gamma_enabled = 1 if conf.gamma_correction_enabled else 0
     p.chunk_paths = conf.chunk_paths
+    p.renderer = conf.renderer
+    p.max_visibility = conf.max_visibility
+    p.shadow_intensity = conf.shadow_intensity
     return p
 
 

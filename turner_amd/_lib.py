@@ -35,6 +35,9 @@ TN_RENDER_GUIDED = 8
 TN_RENDER_EXHAUSTIVE = 16
 TN_RENDER_GRAPH = 32
 TN_RENDER_AUTOTUNE = 64
+TN_RENDERER_PATHTRACER = 0
+TN_RENDERER_RAYCASTER = 1
+TN_RENDERER_RAYTRACER = 2
 
 
 class TurnerError(RuntimeError):
@@ -65,6 +68,7 @@ class RenderParams(C.Structure):
                 ("seed", C.c_uint64), ("bg", C.c_float * 4), ("exposure", C.c_float),
                 ("inverse_gamma", C.c_float), ("gamma_enabled", C.c_int32),
                 ("chunk_paths", C.c_uint32), ("sample_offset", C.c_uint32),
+                ("renderer", C.c_uint32), ("max_visibility", C.c_float), ("shadow_intensity", C.c_float),
                 ("reserved", C.c_uint32)]
 
 
@@ -92,6 +96,7 @@ SIGNATURES = {
     "tn_scene_create": (C.c_int, [_P, _P, _P, _P, _U32, _U32, C.POINTER(_P)]),
     "tn_scene_create_with_tree": (C.c_int, [_P, _P, _P, _P, _U32, _P, C.c_size_t, _P,
                                             C.POINTER(_P)]),
+    "tn_scene_set_reflection": (C.c_int, [_P, _P, _P]),
     "tn_scene_destroy": (C.c_int, [_P]),
     "tn_scene_get_info": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "tn_scene_get_flat_nodes": (C.c_int, [_P, _P, C.c_size_t]),

@@ -37,7 +37,7 @@ class Batch:
             pass
 
     def render(self, jobs, max_depth=3, mc_samples=1, flags=0, seed=1, chunk_paths=0, sample_offset=0,
-               accum=None):
+               accum=None, renderer=0):
         """jobs: one dict per scene with camera, lights (<= 1), width, height,
         spp and optional bg.  Returns (list of (h, w, 4) device tensors of
         per-pixel MEANS, stats dict, accum tensor of the sums)."""
@@ -65,6 +65,7 @@ class Batch:
         p = _lib.RenderParams()
         p.max_depth, p.mc_samples, p.flags, p.seed = max_depth, mc_samples, flags, seed
         p.chunk_paths, p.sample_offset = chunk_paths, sample_offset
+        p.renderer = renderer  # batches render with the path tracer only
         st = _lib.RenderStats()
         stream = torch.cuda.current_stream(accum.device).cuda_stream
         check(lib().tn_batch_render(self.handle, J, C.byref(p), ptr(accum), C.byref(st), stream))

@@ -234,6 +234,8 @@ void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_emission);
     (void)hipFree(sc->d_emit_ids);
     (void)hipFree(sc->d_emit_cdf);
+    (void)hipFree(sc->d_reflective);
+    (void)hipFree(sc->d_reflectivity);
     (void)hipFree(sc->d_counters);
     (void)hipFree(sc->d_status);
 }
@@ -472,6 +474,19 @@ int tn_scene_create_with_tree(const float* verts9, const float* normals9, const 
         return set_error(TN_ERR_OOM, "host allocation failed");
     }
     *out = sc.release();
+    return TN_OK;
+}
+
+int tn_scene_set_reflection(tn_scene* sc, const float* reflective4, const float* reflectivity) {
+    if (!sc || !reflective4 || !reflectivity) return set_error(TN_ERR_INVALID, "NULL argument");
+    for (uint32_t i = 0; i < sc->ntris; ++i)
+        if (!std::isfinite(reflectivity[i])) return set_error(TN_ERR_INVALID, "reflectivity %u is not finite", i);
+    TN_HIP(hipSetDevice(sc->device));
+    int rc;
+    if (!sc->d_reflective && (rc = dev_upload(&sc->d_reflective, (const float4*)nullptr, sc->ntris))) return rc;
+    if (!sc->d_reflectivity && (rc = dev_upload(&sc->d_reflectivity, (const float*)nullptr, sc->ntris))) return rc;
+    TN_HIP(hipMemcpy(sc->d_reflective, reflective4, (size_t)sc->ntris * 16, hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(sc->d_reflectivity, reflectivity, (size_t)sc->ntris * 4, hipMemcpyHostToDevice));
     return TN_OK;
 }
 

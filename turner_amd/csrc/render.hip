@@ -74,7 +74,8 @@ struct Queues {
     uint32_t cap;   // capacity of each path queue (entries)
     uint32_t scap;  // capacity of the shadow queue
     // device counters: [0..kMaxLevels) path counts, [kMaxLevels..2k) shadow
-    // counts, [2k..3k) trace work, [3k..4k) shadow work, [4k] overflow
+    // counts, [2k..3k) trace work, [3k..4k) shadow work, [4k] overflow,
+    // [4k + 1] raycaster hits
     uint32_t* cnt;
     unsigned long long* totals;  // [0] segments, [1] shadow rays
 };
@@ -133,6 +134,11 @@ struct RenderArgs {
     uint32_t* ts_count;
     uint32_t ts_cap;
     float accept;  // probability that a record enters the training set
+    // raycaster / raytracer
+    float max_vis;    // raycaster max_visibility
+    float shadow_si;  // raytracer shadow_intensity
+    const float4* reflective;   // raytracer: reflective colour per triangle
+    const float* reflectivity;  // raytracer: null = no reflection
 };
 
 __device__ __forceinline__ uint32_t scene_of_pixel(const RenderArgs& A, uint32_t pixel) {
@@ -204,7 +210,7 @@ __global__ void k_chunk_begin(uint32_t* cnt, ChunkState* cs) {
     const unsigned long long first = cs->next;
     const unsigned long long left = cs->total > first ? cs->total - first : 0ull;
     const uint32_t count = (uint32_t)(left < cs->chunk ? left : cs->chunk);
-    if (i < 4 * kMaxLevels + 1) cnt[i] = (i == 0) ? count : 0u;
+    if (i < 4 * kMaxLevels + 2) cnt[i] = (i == 0) ? count : 0u;
     __syncthreads();
     if (i == 0) {
         cs->first = first;
@@ -218,9 +224,10 @@ __global__ void k_zero_totals(unsigned long long* totals, uint32_t* incomplete) 
     if (threadIdx.x == 2) *incomplete = 0;
 }
 
+// levels < 0: the raycaster, whose rays are its hits (cnt[4 kMaxLevels + 1])
 __global__ void k_chunk_totals(const uint32_t* cnt, unsigned long long* totals, int levels) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        unsigned long long a = 0, b = 0;
+        unsigned long long a = levels < 0 ? cnt[4 * kMaxLevels + 1] : 0, b = 0;
         for (int k = 0; k < levels; ++k) {
             a += cnt[k];
             b += cnt[kMaxLevels + k];
@@ -383,6 +390,190 @@ __device__ __forceinline__ void push_shadow(const Queues& Q, uint32_t k, float3 
     Q.sd[k] = make_float4(dir.x, dir.y, dir.z, tmax);
     Q.sc[k] = contrib;
     if (records) Q.sx[k] = make_float4(local.x, local.y, local.z, __uint_as_float(slot));
+}
+
+// ------------------------------------------------ raycaster and raytracer
+// The reference's other two trace() plugins on the same queues and traversal.
+
+// Raycaster shade (raycaster.cpp:7-24), level 0 only: a miss gives the
+// background; a hit the triangle's diffuse colour with alpha
+// clamp(1 - r / max_visibility, 0, 1).  Hits are counted in cnt[4 kMaxLevels + 1]
+// (the reference counts num_rays on hits only).
+__global__ __launch_bounds__(kRBlock) void k_cast(RenderArgs A, Queues Q, int level, int buf) {
+    const uint32_t n = min(Q.cnt[level], Q.cap);
+    uint32_t hits = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 qo = Q.o[buf][i];
+        const float4 w = Q.w[buf][i];
+        const uint32_t pixel = __float_as_uint(qo.w);
+        const SceneParams& S = A.sp[scene_of_pixel(A, pixel)];
+        const uint4 h = Q.hit[i];
+        if (h.x == kInvalidId) {
+            accumulate(A.accum, pixel, mul4(w, S.bg));
+            continue;
+        }
+        ++hits;
+        float4 c = A.albedo[h.x];
+        const float a = 1.f - __uint_as_float(h.y) / A.max_vis;
+        c.w = a < 0.f ? 0.f : (a > 1.f ? 1.f : a);  // clamp of src/turner.h:28-36
+        accumulate(A.accum, pixel, mul4(w, c));
+    }
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off, 64);
+    if ((threadIdx.x & 63u) == 0 && hits) atomicAdd(Q.cnt + 4 * kMaxLevels + 1, hits);
+}
+
+// Raytracer shade (raytracer.cpp:6-67) of one level.  A segment carries the
+// product W of the factors reflectivity * reflective (and shadow darkening)
+// of its ancestors.  A hit contributes W * colour, colour = Lambert
+// max(0, L.n) diffuse light (lib/lambertian.h:15-19), times (1 - reflectivity)
+// when it reflects, and spawns the mirror ray d - 2 (n.d) n from p2 with
+// weight W * reflectivity * reflective.  The contribution waits in the shadow
+// queue with the child's index: k_rt_shadow darkens both by the shadow
+// intensity when the light is blocked (raytracer.cpp:57-64).  Children beyond
+// max_depth are counted (the reference counts the call that returns {})
+// but not queued.
+__global__ __launch_bounds__(kRBlock) void k_rt_shade(RenderArgs A, Queues Q, int level, int buf) {
+    const uint32_t n = min(Q.cnt[level], Q.cap);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int ob = buf ^ 1;
+    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
+        const uint32_t i = base + lane;
+        bool has_hit = false, spawn = false, shadow = false;
+        float3 p2 = make_float3(0, 0, 0), rdir = p2, sdir = p2;
+        float sdist = 0.f;
+        float4 contrib = make_float4(0, 0, 0, 0), cw = contrib;
+        uint32_t pixel = 0;
+        float pathbits = 0.f;
+        if (i < n) {
+            const float4 qo = Q.o[buf][i], qd = Q.d[buf][i];
+            const float4 w = Q.w[buf][i];
+            pixel = __float_as_uint(qo.w);
+            pathbits = qd.w;
+            const SceneParams& S = A.sp[scene_of_pixel(A, pixel)];
+            const uint4 h = Q.hit[i];
+            if (h.x == kInvalidId) {
+                accumulate(A.accum, pixel, mul4(w, S.bg));  // raytracer.cpp:19-21
+            } else {
+                has_hit = true;
+                const uint32_t id = h.x;
+                const float r = __uint_as_float(h.y), s = __uint_as_float(h.z), t = __uint_as_float(h.w);
+                const float3 p = make_float3(qo.x + r * qd.x, qo.y + r * qd.y, qo.z + r * qd.z);
+                // raytracer.cpp:24-26: light direction from p
+                const float3 l = make_float3(S.light_pos.x - p.x, S.light_pos.y - p.y, S.light_pos.z - p.z);
+                const float il = 1.0f / sqrtf(l.x * l.x + l.y * l.y + l.z * l.z);
+                const float3 ld = make_float3(il * l.x, il * l.y, il * l.z);
+                // raytracer.cpp:29-30 (lib/triangle.h:54-56)
+                const float* nv = A.normals + 9 * (size_t)id;
+                const float b0 = 1.f - s - t;
+                const float3 ni = make_float3(b0 * nv[0] + s * nv[3] + t * nv[6], b0 * nv[1] + s * nv[4] + t * nv[7],
+                                              b0 * nv[2] + s * nv[5] + t * nv[8]);
+                const float inv = 1.0f / sqrtf(ni.x * ni.x + ni.y * ni.y + ni.z * ni.z);
+                const float3 nn = make_float3(inv * ni.x, inv * ni.y, inv * ni.z);
+                // raytracer.cpp:33-34
+                const float cosa = fmaxf(0.f, ld.x * nn.x + ld.y * nn.y + ld.z * nn.z);
+                const float4 direct = mul4(scale4(cosa, A.albedo[id]), S.light_col);
+                // raytracer.cpp:37
+                p2 = make_float3(p.x + nn.x * 0.0001f, p.y + nn.y * 0.0001f, p.z + nn.z * 0.0001f);
+                float4 color = direct;
+                const float refl = A.reflectivity ? A.reflectivity[id] : 0.f;
+                if (refl > 0) {  // raytracer.cpp:42-52
+                    const float dn = 2.f * (nn.x * qd.x + nn.y * qd.y + nn.z * qd.z);
+                    rdir = make_float3(qd.x - dn * nn.x, qd.y - dn * nn.y, qd.z - dn * nn.z);
+                    color = scale4(1.f - refl, direct);
+                    cw = mul4(w, scale4(refl, A.reflective[id]));
+                    spawn = true;
+                }
+                contrib = mul4(w, color);
+                // raytracer.cpp:55-60: shadow ray from p2 toward the light
+                const float3 l2 = make_float3(S.light_pos.x - p2.x, S.light_pos.y - p2.y, S.light_pos.z - p2.z);
+                sdist = sqrtf(l2.x * l2.x + l2.y * l2.y + l2.z * l2.z);
+                const float il2 = 1.0f / sdist;
+                sdir = make_float3(il2 * l2.x, il2 * l2.y, il2 * l2.z);
+                shadow = sdist > 0.f;
+            }
+        }
+        // children (mirror rays)
+        const uint64_t mc = __ballot(spawn);
+        uint32_t child = ~0u;
+        if (mc) {
+            uint32_t cbase = 0;
+            if (lane == 0) cbase = atomicAdd(&Q.cnt[level + 1], (uint32_t)__popcll(mc));
+            cbase = __shfl(cbase, 0);
+            if (spawn && level < A.max_depth) {
+                const uint32_t k = cbase + lanes_below(mc);
+                if (k < Q.cap) {
+                    Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
+                    Q.d[ob][k] = make_float4(rdir.x, rdir.y, rdir.z, pathbits);
+                    Q.w[ob][k] = cw;
+                    child = k;
+                } else {
+                    atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
+                }
+            }
+        }
+        // lit test; a hit whose light sits at p2 itself is lit
+        if (has_hit && !shadow) accumulate(A.accum, pixel, contrib);
+        const uint64_t ms = __ballot(shadow);
+        if (ms) {
+            uint32_t sbase = 0;
+            if (lane == 0) sbase = atomicAdd(&Q.cnt[kMaxLevels + level], (uint32_t)__popcll(ms));
+            sbase = __shfl(sbase, 0);
+            if (shadow) {
+                const uint32_t k = sbase + lanes_below(ms);
+                if (k < Q.scap) {
+                    // shadowed iff the closest hit is nearer than the light
+                    // (strict): any hit with r <= the float below sdist
+                    const float tmax = __uint_as_float(__float_as_uint(sdist) - 1u);
+                    Q.so[k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
+                    Q.sd[k] = make_float4(sdir.x, sdir.y, sdir.z, tmax);
+                    Q.sc[k] = contrib;
+                    Q.sx[k] = make_float4(__uint_as_float(child), 0.f, 0.f, 0.f);
+                } else {
+                    atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
+                }
+            }
+        }
+    }
+}
+
+// Raytracer shadow test (raytracer.cpp:55-64): a blocked light darkens the
+// segment's colour, and so the weight of its mirror child, by
+// c -= c * shadow_intensity.
+template <bool EXH>
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_rt_shadow(
+    RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ unsigned long long slots[kRBlock];
+    const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
+    const int ob = buf ^ 1;
+    TraversalCounters cnt;
+    coop_loop<kStack, EXH, false, true, false>(
+        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
+        [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
+            const float4 a = Q.so[k], b = Q.sd[k];
+            o = make_float3(a.x, a.y, a.z);
+            d = make_float3(b.x, b.y, b.z);
+            tmax = b.w;
+            scene = scene_of_pixel(A, __float_as_uint(a.w));
+        },
+        [&](uint32_t k, const Trav& T, bool traversed) {
+            float4 c = Q.sc[k];
+            if (traversed && T.best_id != kInvalidId) {
+                const float si = A.shadow_si;
+                c = make_float4(c.x - c.x * si, c.y - c.y * si, c.z - c.z * si, c.w - c.w * si);
+                const uint32_t child = __float_as_uint(Q.sx[k].x);
+                if (child != ~0u) {
+                    const float4 w = Q.w[ob][child];
+                    Q.w[ob][child] = make_float4(w.x - w.x * si, w.y - w.y * si, w.z - w.z * si, w.w - w.w * si);
+                }
+            }
+            accumulate(A.accum, __float_as_uint(Q.so[k].w), c);
+        },
+        cnt);
+    if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
 }
 
 // Shade one level.  GUIDED kernels run one wave per block (the guide's
@@ -775,7 +966,7 @@ size_t carve(Queues& q, ChunkState** cs, char* base, uint32_t cap) {
     q.sd = (float4*)take((size_t)q.scap * 16);
     q.sc = (float4*)take((size_t)q.scap * 16);
     q.sx = (float4*)take((size_t)q.scap * 16);
-    q.cnt = (uint32_t*)take((4 * kMaxLevels + 1) * sizeof(uint32_t));
+    q.cnt = (uint32_t*)take((4 * kMaxLevels + 2) * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
     *cs = (ChunkState*)take(sizeof(ChunkState));
     q.cap = cap;
@@ -839,6 +1030,8 @@ struct RenderGeom {
     const float4* trirec;
     const uint32_t* emit_ids;
     const float* emit_cdf;
+    const float4* reflective = nullptr;
+    const float* reflectivity = nullptr;
     int device;
     int cus;
     uint32_t* status;  // 4 words; [2] counts incomplete render traversals
@@ -853,6 +1046,8 @@ RenderGeom geom_of(const tn_scene* sc) {
     g.trirec = sc->d_tris;
     g.emit_ids = sc->d_emit_ids;
     g.emit_cdf = sc->d_emit_cdf;
+    g.reflective = sc->d_reflective;
+    g.reflectivity = sc->d_reflectivity;
     g.device = sc->device;
     g.cus = sc->cus;
     g.status = sc->d_status;
@@ -933,8 +1128,21 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         return set_error(TN_ERR_INVALID, "training records need mc_samples == 1 (one path per sample)");
     if (guided && guide->device != G.device)
         return set_error(TN_ERR_INVALID, "guide and scene live on different devices");
+    const uint32_t renderer = P->renderer;
+    if (renderer > TN_RENDERER_RAYTRACER) return set_error(TN_ERR_INVALID, "unknown renderer %u", renderer);
+    if (renderer != TN_RENDERER_PATHTRACER && guided)
+        return set_error(TN_ERR_INVALID, "guided rendering needs the path tracer");
+    if (renderer == TN_RENDERER_RAYTRACER)
+        for (const SceneParams& S : table)
+            if (S.nlights != 1)
+                return set_error(TN_ERR_INVALID, "the raytracer needs one point light (raytracer.cpp:14)");
+    // queue sizing: the raycaster traces one level, the raytracer spawns at
+    // most one (mirror) child per hit
+    tn_render_params q = *P;
+    if (renderer == TN_RENDERER_RAYCASTER) q.max_depth = 0;
+    if (renderer != TN_RENDERER_PATHTRACER) q.mc_samples = 1;
     uint32_t chunk = 0, cap = 0;
-    if ((rc = chunk_size(P, shape.chunk, chunk, cap))) return rc;
+    if ((rc = chunk_size(&q, shape.chunk, chunk, cap))) return rc;
     if (st) *st = tn_render_stats{};
     if (total == 0) return TN_OK;
 
@@ -956,8 +1164,12 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     A.nscenes = (uint32_t)table.size();
     A.cs = rb.cs;
     A.status = G.status;
-    A.max_depth = P->max_depth;
+    A.max_depth = q.max_depth;
     A.mc = P->mc_samples;
+    A.max_vis = P->max_visibility;
+    A.shadow_si = P->shadow_intensity;
+    A.reflective = G.reflective;
+    A.reflectivity = G.reflectivity;
     A.flags = P->flags;
     A.seed = P->seed;
     A.accum = (float4*)accum;
@@ -994,7 +1206,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals, G.status + 2);
     k_chunk_setup<<<1, 64, 0, stream>>>(rb.cs, total, chunk);
     const bool exh = P->flags & TN_RENDER_EXHAUSTIVE;
-    const int levels = P->max_depth + 1;
+    const int levels = q.max_depth + 1;
     const uint32_t bpc = std::max(1u, shape.blocks_per_cu);
     const dim3 tgrid((uint32_t)G.cus * bpc), tblock(kRBlock);
     const uint32_t rgrid = (chunk + kRBlock - 1) / kRBlock;
@@ -1008,13 +1220,26 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         for (int lv = 0; lv < levels; ++lv) {
             if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
             else k_trace<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
-            else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            if (exh) k_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv);
-            else k_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv);
+            if (renderer == TN_RENDERER_RAYCASTER) {
+                k_cast<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            } else if (renderer == TN_RENDERER_RAYTRACER) {
+                k_rt_shade<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+                if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+                else k_rt_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            } else {
+                if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
+                else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+                if (exh) k_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv);
+                else k_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv);
+            }
             buf ^= 1;
         }
-        k_chunk_totals<<<1, 64, 0, s>>>(Q.cnt, Q.totals, levels);
+        // the raytracer also counts the mirror rays of its last level
+        // (trace() is entered and returns {} beyond max_depth)
+        k_chunk_totals<<<1, 64, 0, s>>>(Q.cnt, Q.totals,
+                                         renderer == TN_RENDERER_RAYCASTER   ? -1
+                                         : renderer == TN_RENDERER_RAYTRACER ? levels + 1
+                                                                             : levels);
         if (training) k_collect<<<rgrid, kRBlock, 0, s>>>(A, P->max_depth);
     };
     const uint64_t nchunks = (total + chunk - 1) / chunk;
@@ -1283,9 +1508,16 @@ int tn_trace(tn_scene* sc, const float* o3, const float* d3, uint32_t n, int32_t
     if (!sc || !P || (n && (!o3 || !d3 || !rgba)) || (nl && !lights) || depth < 0)
         return set_error(TN_ERR_INVALID, "NULL or invalid argument");
     if (st) *st = tn_render_stats{};
-    if (n == 0 || depth > P->max_depth) return TN_OK;  // pathtracer.cpp:17-19
+    if (n == 0) return TN_OK;
     tn_render_params q = *P;
-    q.max_depth = P->max_depth - depth;  // levels depth..max_depth remain
+    if (P->renderer == TN_RENDERER_RAYCASTER) {
+        q.max_depth = 0;  // raycaster.cpp:7-24 ignores the depth
+    } else if (depth > P->max_depth) {  // pathtracer.cpp:17-19, raytracer.cpp:9-13
+        if (st && P->renderer == TN_RENDERER_RAYTRACER) st->rays = n;  // counted before the return
+        return TN_OK;
+    } else {
+        q.max_depth = P->max_depth - depth;  // levels depth..max_depth remain
+    }
     return render_into(sc, nullptr, lights, nl, &q, rgba, st, (hipStream_t)stream, o3, d3, n);
 }
 
@@ -1424,6 +1656,7 @@ int tn_batch_render(tn_batch* b, const tn_batch_job* jobs, const tn_render_param
     if (!b || !jobs || !P || !accum) return set_error(TN_ERR_INVALID, "NULL argument");
     if (P->flags & (TN_RENDER_GUIDED | TN_RENDER_REF_JITTER))
         return set_error(TN_ERR_INVALID, "batches render without guide or reference jitter");
+    if (P->renderer != TN_RENDERER_PATHTRACER) return set_error(TN_ERR_INVALID, "batches render with the path tracer");
     int rc = check_render_params(P);
     if (rc) return rc;
     TN_HIP(hipSetDevice(b->device));

@@ -30,6 +30,8 @@ struct tn_scene {
     float4* d_emission = nullptr;
     uint32_t* d_emit_ids = nullptr;
     float* d_emit_cdf = nullptr;  // inclusive prefix of emitter areas
+    float4* d_reflective = nullptr;    // raytracer: reflective colour (null: no reflection)
+    float* d_reflectivity = nullptr;   // raytracer: reflectivity factor
     unsigned long long* d_counters = nullptr;  // 8 counters
     uint32_t* d_status = nullptr;              // [0] incomplete traversals, [1] queue overflow
 
