@@ -131,3 +131,75 @@ def test_trace_batch_matches_render(gpu):
     torch.cuda.synchronize()
     assert float(rgba.abs().sum()) == 0.0
     assert outs[0][..., :3].mean() > 0.01
+
+
+RAYCASTER = os.path.join(ROOT, "turner_amd", "raycaster")
+RAYTRACER = os.path.join(ROOT, "turner_amd", "raytracer")
+
+
+def run_exe(exe, *args):
+    return subprocess.run([exe, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+
+
+def test_plugin_front_ends_config():
+    """raycaster.h:4-22 / raytracer.h:4-22 options and config.h:110-124 checks."""
+    p = run_exe(RAYCASTER, "--help")
+    assert p.returncode == 0 and p.stdout.startswith("Usage: raycaster <filename>")
+    assert "--max-visibility=<float>" in p.stdout
+    p = run_exe(RAYTRACER, "--help")
+    assert p.stdout.startswith("Usage: raytracer <filename>") and "--shadow=<float>" in p.stdout
+    c = config(run_exe(RAYCASTER, "f", "--print-config"))
+    assert c["Max visibility"] == "2"
+    c = config(run_exe(RAYCASTER, "f", "--max-visibility=3.5", "--print-config"))
+    assert c["Max visibility"] == "3.5"
+    c = config(run_exe(RAYTRACER, "f", "--print-config"))
+    assert (c["Max recursion depth"], c["Shadow intensity"]) == ("3", "0.5")
+    c = config(run_exe(RAYTRACER, "f", "-d", "5", "--shadow=0.25", "--print-config"))
+    assert (c["Max recursion depth"], c["Shadow intensity"]) == ("5", "0.25")
+    assert run_exe(RAYTRACER, "f", "--shadow=1.5", "--print-config").returncode != 0
+    assert run_exe(RAYCASTER, "f", "--max-visibility=-1", "--print-config").returncode != 0
+
+
+def _ppm_ints(text):
+    lines = text.strip().split("\n")
+    return lines[:3], np.array([int(x) for x in " ".join(lines[3:]).split()])
+
+
+def _display_ppm_ints(lin, exposure=1.0, inverse_gamma=0.454545):
+    """main.cpp:216-223 then lib/raster.h:79-100 in float32: exposure and gamma
+    on RGB, alpha multiplied in when writing."""
+    rgb = lin[..., :3].astype(np.float32)
+    rgb = np.float32(1) - np.exp(-rgb * np.float32(exposure))
+    rgb = np.power(rgb, np.float32(inverse_gamma))
+    c = np.float32(255) * rgb * lin[..., 3:4]
+    return np.clip(c, 0, 255).astype(np.int64).reshape(-1)
+
+
+@pytest.mark.gpu
+def test_cli_raycaster_and_raytracer_match_oracle(gpu, tmp_path):
+    """The raytracer front end on a .tns v2 scene with mirrors and the
+    raycaster front end: PPM equal to the oracle's image through exposure,
+    gamma and the alpha-weighted writer (within one level for libm rounding)."""
+    import oracle as O
+    from tests.test_gpu_plugins import _mirror_box
+    sc = _mirror_box()
+    path = str(tmp_path / "mirror.tns")
+    scenes.save_tns(sc, path)
+    w = 48
+    ot = O.Tree(sc["verts"])
+    for exe, renderer, extra, kw in ((RAYTRACER, O.RENDERER_RAYTRACER, ["-d", "3", "--shadow=0.4"],
+                                      {"shadow_intensity": 0.4}),
+                                     (RAYCASTER, O.RENDERER_RAYCASTER, ["--max-visibility=2000"],
+                                      {"max_visibility": 2000.0})):
+        p = run_exe(exe, path, "-w", str(w), "--background", "0.1 0.2 0.3", *extra)
+        assert p.returncode == 0, p.stderr
+        head, pix = _ppm_ints(p.stdout)
+        assert head == ["P3", f"{w} {w}", "255"] and pix.size == w * w * 3
+        lin, _, nr, _ = O.render(sc, ot, w, w, 3, 1, 1, flags=O.FLAG_REF_RNG, bg=(0.1, 0.2, 0.3, 1.0),
+                                 nthreads=1, renderer=renderer, **kw)
+        want = _display_ppm_ints(lin)
+        diff = np.abs(pix - want)
+        print(f"[cli {os.path.basename(exe)}] max |d| {diff.max()}, exact {np.mean(diff == 0):.4f}", flush=True)
+        assert diff.max() <= 1 and np.mean(diff == 0) > 0.97
+        stats = dict(re.findall(r"^(\w[\w ()/-]*?)\s*: (\S+)", p.stderr, re.M))
+        assert int(stats["Rays"]) == nr

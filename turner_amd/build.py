@@ -73,17 +73,26 @@ def build_oracle():
 
 
 def build_cli():
-    """Host C++ pathtracer front end (reference CLI flags) over the C ABI."""
+    """Host C++ front ends over the C ABI, one per trace() plugin as the
+    reference links them: pathtracer, raycaster, raytracer (same flags)."""
     host = os.path.join(HERE, "host")
-    out = os.path.join(HERE, "pathtracer")
     srcs = [os.path.join(host, f) for f in ("pathtracer_main.cpp", "turner.cpp")]
     if not all(os.path.exists(s) for s in srcs):
         return None
-    _run(["g++", "-O2", "-std=c++17", "-pthread", "-D__HIP_PLATFORM_AMD__",
-          "-I", os.path.join(ROOT, "include"), "-I", host, "-I", os.path.join(ROCM, "include"),
-          *srcs, "-o", out, f"-L{HERE}", "-lturner_amd", f"-L{os.path.join(ROCM, 'lib')}",
-          "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"])
-    return out
+    os.makedirs(BUILD, exist_ok=True)
+    inc = ["-O2", "-std=c++17", "-pthread", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+           "-I", host, "-I", os.path.join(ROCM, "include")]
+    lib_obj = os.path.join(BUILD, "turner_host.o")
+    _run(["g++", *inc, "-c", srcs[1], "-o", lib_obj])
+    link = [f"-L{HERE}", "-lturner_amd", f"-L{os.path.join(ROCM, 'lib')}", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+            f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+    jobs = []
+    for name, renderer in (("pathtracer", 0), ("raycaster", 1), ("raytracer", 2)):
+        out = os.path.join(HERE, name)
+        jobs.append(["g++", *inc, f"-DTN_CLI_RENDERER={renderer}", srcs[0], lib_obj, "-o", out, *link])
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        list(ex.map(_run, jobs))
+    return os.path.join(HERE, "pathtracer")
 
 
 if __name__ == "__main__":

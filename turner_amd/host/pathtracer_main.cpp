@@ -1,8 +1,11 @@
-// pathtracer_main.cpp -- the `pathtracer` front end (main.cpp:88-244 with the
-// USAGE of pathtracer.h:4-25) over the MI355X renderer.  Scene files are the
-// turner_amd .tns format (the reference's assimp .blend import is not
-// available); the image goes to stdout as PPM P3, statistics to stderr in the
-// reference's format (lib/output.h:101-112).
+// pathtracer_main.cpp -- the `pathtracer`, `raycaster` and `raytracer` front
+// ends (main.cpp:88-244 with the USAGE of pathtracer.h:4-25, raycaster.h:4-22
+// and raytracer.h:4-22) over the MI355X renderer; TN_CLI_RENDERER picks the
+// trace() plugin at compile time, as the reference links one renderer per
+// binary (CMakeLists.txt:62-67).  Scene files are the turner_amd .tns format
+// (the reference's assimp .blend import is not available); the image goes to
+// stdout as PPM P3, statistics to stderr in the reference's format
+// (lib/output.h:101-112).
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -15,11 +18,16 @@
 
 #include "turner.hpp"
 
+#ifndef TN_CLI_RENDERER
+#define TN_CLI_RENDERER TN_RENDERER_PATHTRACER
+#endif
+
 namespace {
 
-const char* kUsage =
-    R"(Usage: pathtracer <filename> [options]
+constexpr uint32_t kRenderer = TN_CLI_RENDERER;
 
+const char* kCommon =
+    R"(
 Options:
   -w --width=<px>                   Width of the image [default: 640].
   -a --aspect=<num>                 Aspect ratio of the image. If the model has
@@ -32,7 +40,10 @@ Options:
   --no-gamma-correction             Disables gamma correction.
   --exposure=<float>                Exposure [default: 1].
   -v --verbose                      Verbose output.
+)";
 
+const char* kPathtracer =
+    R"(
 Pathtracer options:
   -d --max-depth=<int>              Maximum recursion depth for raytracing
                                     [default: 3].
@@ -53,6 +64,45 @@ MI355X options:
   --train-passes=<int>              Guiding: training passes of 2 spp [default: 16].
   --print-config                    Print the parsed configuration and exit.
 )";
+
+const char* kRaycaster =
+    R"(
+Raycaster options:
+  --max-visibility=<float>   Any object farther away is dark [default: 2.0].
+
+MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per row):
+  -p --pixel-samples=<int>          Number of samples per pixel [default: 1].
+  --exhaustive                      Reference traversal order without early exit.
+  --tree-cache=<path>               kd-tree cache file (validated against the scene).
+  --render-tree                     Rendering kd-tree (same closest hits).
+  --graph                           Replay the per-chunk launch sequence from a hipGraph.
+  --print-config                    Print the parsed configuration and exit.
+)";
+
+const char* kRaytracer =
+    R"(
+Raytracer options:
+  -d --max-depth=<int>      Maximum recursion depth for raytracing [default: 3].
+  --shadow=<float>          Intensity of shadow [default: 0.5].
+
+MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per row):
+  -p --pixel-samples=<int>          Number of samples per pixel [default: 1].
+  --exhaustive                      Reference traversal order without early exit.
+  --tree-cache=<path>               kd-tree cache file (validated against the scene).
+  --render-tree                     Rendering kd-tree (same closest hits).
+  --graph                           Replay the per-chunk launch sequence from a hipGraph.
+  --print-config                    Print the parsed configuration and exit.
+)";
+
+std::string usage() {
+    const char* name = kRenderer == TN_RENDERER_RAYCASTER   ? "raycaster"
+                       : kRenderer == TN_RENDERER_RAYTRACER ? "raytracer"
+                                                            : "pathtracer";
+    const char* own = kRenderer == TN_RENDERER_RAYCASTER   ? kRaycaster
+                      : kRenderer == TN_RENDERER_RAYTRACER ? kRaytracer
+                                                           : kPathtracer;
+    return std::string("Usage: ") + name + " <filename> [options]\n" + kCommon + own;
+}
 
 struct Args {
     std::map<std::string, std::string> opt;
@@ -131,7 +181,7 @@ int main(int argc, char** argv) {
     try {
         Args a = parse(argc, argv);
         if (a.opt.count("--help") || a.pos.size() != 1) {
-            std::cout << kUsage;
+            std::cout << usage();
             return a.opt.count("--help") ? 0 : 1;
         }
         conf.filename = a.pos[0];
@@ -146,6 +196,13 @@ int main(int argc, char** argv) {
         conf.max_recursion_depth = std::stoi(get(a, "--max-depth", "3"));
         conf.num_pixel_samples = std::stoi(get(a, "--pixel-samples", "1"));
         conf.num_monte_carlo_samples = std::stoi(get(a, "--monte-carlo-samples", "8"));
+        conf.renderer = kRenderer;
+        conf.max_visibility = std::stof(get(a, "--max-visibility", "2.0"));
+        conf.shadow_intensity = std::stof(get(a, "--shadow", "0.5"));
+        if (kRenderer != TN_RENDERER_PATHTRACER) {
+            conf.num_monte_carlo_samples = 1;
+            conf.flags |= TN_RENDER_REF_JITTER;
+        }
         conf.seed = std::stoull(get(a, "--seed", "1"));
         conf.tree_cache = get(a, "--tree-cache", "");
         if (a.opt.count("--cosine")) conf.flags |= TN_RENDER_COSINE;
@@ -154,14 +211,15 @@ int main(int argc, char** argv) {
         if (a.opt.count("--exhaustive")) conf.flags |= TN_RENDER_EXHAUSTIVE;
         if (a.opt.count("--graph")) conf.flags |= TN_RENDER_GRAPH;
         render_tree = a.opt.count("--render-tree") > 0;
-        guided = a.opt.count("--guided") > 0;
+        guided = kRenderer == TN_RENDERER_PATHTRACER && a.opt.count("--guided") > 0;
         train_passes = std::stoi(get(a, "--train-passes", "16"));
         if (guided && (conf.num_monte_carlo_samples != 1 || train_passes < 0))
             throw std::runtime_error("--guided needs -m 1 and --train-passes >= 0");
         // config.h:29-33,119-126
         if (!(0 < conf.aspect && 1 <= conf.num_threads && 0 <= conf.exposure &&
               0 < conf.max_recursion_depth && 1 <= conf.num_pixel_samples &&
-              0 <= conf.num_monte_carlo_samples))
+              0 <= conf.num_monte_carlo_samples && 0 <= conf.max_visibility &&
+              0 <= conf.shadow_intensity && conf.shadow_intensity <= 1))
             throw std::runtime_error("invalid configuration");
         if (conf.verbose || a.opt.count("--print-config")) {
             std::cerr << "Filename: " << conf.filename << "\nAspect ratio: " << conf.aspect
@@ -170,13 +228,14 @@ int main(int argc, char** argv) {
                       << "\n  Exposure: " << conf.exposure << "\n  Gamma correction enabled: "
                       << conf.gamma_correction_enabled
                       << "\n\nTracer parameters (not all applicable):\n  Max recursion depth: "
-                      << conf.max_recursion_depth << "\n  Number of pixel samples: "
+                      << conf.max_recursion_depth << "\n  Max visibility: " << conf.max_visibility
+                      << "\n  Shadow intensity: " << conf.shadow_intensity << "\n  Number of pixel samples: "
                       << conf.num_pixel_samples << "\n  Number of Monte-Carlo samples: "
                       << conf.num_monte_carlo_samples << std::endl;
             if (a.opt.count("--print-config")) return 0;
         }
     } catch (const std::exception& e) {
-        std::cerr << e.what() << "\n" << kUsage;
+        std::cerr << e.what() << "\n" << usage();
         return 1;
     }
     try {

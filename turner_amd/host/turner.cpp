@@ -72,6 +72,9 @@ tn_render_params params_of(const TracerConfig& conf, size_t width, size_t height
     p.exposure = conf.exposure;
     p.inverse_gamma = conf.inverse_gamma;
     p.gamma_enabled = conf.gamma_correction_enabled ? 1 : 0;
+    p.renderer = conf.renderer;
+    p.max_visibility = conf.max_visibility;
+    p.shadow_intensity = conf.shadow_intensity;
     return p;
 }
 
@@ -120,6 +123,16 @@ void KDTree::create(const uint64_t* nodes, size_t nn, const float* box, uint32_t
     else
         check(tn_scene_create(v.data(), n.data(), a.data(), e.data(), (uint32_t)tris_.size(), flags, &scene_));
     check(tn_scene_get_info(scene_, &info_));
+    bool reflects = false;
+    for (const Triangle& t : tris_) reflects = reflects || t.reflectivity != 0;
+    if (reflects) {
+        std::vector<float> rc(4 * tris_.size()), rf(tris_.size());
+        for (size_t i = 0; i < tris_.size(); ++i) {
+            std::memcpy(&rc[4 * i], &tris_[i].reflective, 16);
+            rf[i] = tris_[i].reflectivity;
+        }
+        check(tn_scene_set_reflection(scene_, rc.data(), rf.data()));
+    }
 }
 
 KDTree::KDTree(KDTree&& o) noexcept : tris_(std::move(o.tris_)), scene_(o.scene_), info_(o.info_) {
@@ -234,14 +247,20 @@ Scene load_tns(const std::string& path) {
     f.read(reinterpret_cast<char*>(&version), 4);
     f.read(reinterpret_cast<char*>(&ntris), 4);
     f.read(reinterpret_cast<char*>(&nlights), 4);
-    if (!f || std::memcmp(magic, "TNSC", 4) != 0 || version != 1)
+    if (!f || std::memcmp(magic, "TNSC", 4) != 0 || (version != 1 && version != 2))
         throw Error(TN_ERR_INVALID, path + " is not a turner_amd scene file");
     std::vector<float> v(9 * (size_t)ntris), n(9 * (size_t)ntris), a(4 * (size_t)ntris),
-        e(4 * (size_t)ntris);
+        e(4 * (size_t)ntris), rc, rf;
     f.read(reinterpret_cast<char*>(v.data()), v.size() * 4);
     f.read(reinterpret_cast<char*>(n.data()), n.size() * 4);
     f.read(reinterpret_cast<char*>(a.data()), a.size() * 4);
     f.read(reinterpret_cast<char*>(e.data()), e.size() * 4);
+    if (version == 2) {  // reflective colour (4N) and reflectivity (N)
+        rc.resize(4 * (size_t)ntris);
+        rf.resize(ntris);
+        f.read(reinterpret_cast<char*>(rc.data()), rc.size() * 4);
+        f.read(reinterpret_cast<char*>(rf.data()), rf.size() * 4);
+    }
     Scene sc;
     float cam[15];
     f.read(reinterpret_cast<char*>(cam), sizeof cam);
@@ -265,6 +284,10 @@ Scene load_tns(const std::string& path) {
         }
         std::memcpy(&t.diffuse, &a[4 * i], 16);
         std::memcpy(&t.emissive, &e[4 * i], 16);
+        if (!rf.empty()) {
+            std::memcpy(&t.reflective, &rc[4 * i], 16);
+            t.reflectivity = rf[i];
+        }
     }
     return sc;
 }

@@ -45,6 +45,8 @@ struct Triangle {  // the parts of lib/triangle.h:95-114 the path reads
     std::array<Vec3, 3> normals;
     Color diffuse{0.8f, 0.8f, 0.8f, 1.f};
     Color emissive;
+    Color reflective;        // raytracer (lib/triangle.h:100-101)
+    float reflectivity = 0;  // raytracer: 0 = no mirror ray
 };
 using Triangles = std::vector<Triangle>;
 
@@ -134,6 +136,10 @@ struct TracerConfig {  // config.h:45-60 + 103-153, pathtracer USAGE defaults
     uint32_t flags = 0;  // TN_RENDER_*
     uint64_t seed = 1;
     std::string tree_cache;
+    // which trace() plugin (TN_RENDERER_*) and its options
+    uint32_t renderer = TN_RENDERER_PATHTRACER;
+    float max_visibility = 2;    // raycaster --max-visibility (config.h:110)
+    float shadow_intensity = 0.5f;  // raytracer --shadow (config.h:113)
 };
 
 // trace.h:23-25: radiance of one ray from recursion level `depth`.
@@ -187,7 +193,8 @@ RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vec
 std::string to_ppm(const std::vector<Color>& image, size_t width, size_t height);
 
 // Scene file written by turner_amd.scenes.save_tns (Python): triangles,
-// vertex normals, RGBA albedo and emission, camera and point lights.
+// vertex normals, RGBA albedo and emission (version 2 adds the reflective
+// colour and reflectivity), camera and point lights.
 struct Scene {
     Triangles triangles;
     Camera camera;

@@ -312,16 +312,23 @@ def reference_furnace(level=3, rho=0.8):
 
 
 def save_tns(scene, path, aspect=1.0):
-    """Writes a scene for the C++ front end (turner_amd/host/pathtracer):
-    "TNSC", version 1, triangle and light counts, verts (9N), normals (9N),
-    albedo (4N), emission (4N) as float32, camera (pos 3, trafo 9, delta_x,
-    delta_y, aspect) and lights (pos 3, rgba 4)."""
+    """Writes a scene for the C++ front ends (turner_amd/pathtracer,
+    raycaster, raytracer): "TNSC", version (1, or 2 when the scene has
+    "reflectivity"), triangle and light counts, verts (9N), normals (9N),
+    albedo (4N), emission (4N) [, reflective (4N), reflectivity (N)] as
+    float32, camera (pos 3, trafo 9, delta_x, delta_y, aspect) and lights
+    (pos 3, rgba 4)."""
     cam = scene["camera"]
     lights = scene.get("lights", [])
+    keys = [("verts", 9), ("normals", 9), ("albedo", 4), ("emission", 4)]
+    version = 1
+    if "reflectivity" in scene:
+        keys += [("reflective", 4), ("reflectivity", 1)]
+        version = 2
     with open(path, "wb") as f:
         f.write(b"TNSC")
-        f.write(np.array([1, len(scene["verts"]), len(lights)], np.uint32).tobytes())
-        for k, cols in (("verts", 9), ("normals", 9), ("albedo", 4), ("emission", 4)):
+        f.write(np.array([version, len(scene["verts"]), len(lights)], np.uint32).tobytes())
+        for k, cols in keys:
             f.write(np.ascontiguousarray(scene[k], np.float32).reshape(-1, cols).tobytes())
         f.write(np.concatenate([np.asarray(cam["pos"], np.float32).ravel(),
                                 np.asarray(cam["trafo"], np.float32).ravel(),
