@@ -439,8 +439,10 @@ def render_throughput(args):
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     cornell = cornell_gpu_vs_cpu(args)
+    plugins = plugin_throughput(args, sc, tree)
     return {
         "cornell_320": cornell,
+        "plugins": plugins,
         "workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
                      "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
                      f"{args.render_spp} pixel samples timed, rendering kd-tree"),
@@ -448,6 +450,42 @@ def render_throughput(args):
         "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
         "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 3),
     }
+
+
+def plugin_throughput(args, sc, tree):
+    """The reference's other trace() plugins on the same mesh (rendering
+    kd-tree), 1920 x 1080, 16 pixel samples: the raycaster (rays = hits, as
+    raycaster.cpp:17 counts) and the raytracer at max-depth 3 with a point
+    light and the icosphere a half mirror (rays = trace() calls, as
+    raytracer.cpp:9 counts; shadow rays on top)."""
+    import numpy as np
+    import torch
+
+    import turner_amd as T
+    n = len(sc["verts"])
+    tree.set_reflection(np.tile(np.array([0.9, 0.9, 0.9, 1.0], np.float32), (n, 1)),
+                        np.where(np.arange(n) >= 10, 0.5, 0.0).astype(np.float32))
+    lights = [((0.5, 0.9, 0.3), (1.0, 1.0, 1.0, 1.0))]
+    w, h = 1920, 1080
+    acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    out = {}
+    for name, renderer in (("raycaster", T.TN_RENDERER_RAYCASTER), ("raytracer", T.TN_RENDERER_RAYTRACER)):
+        conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
+                              num_monte_carlo_samples=1, renderer=renderer, max_visibility=3.0)
+        T.render_accumulate(tree, sc["camera"], lights, conf, acc)  # warm-up
+        conf.num_pixel_samples = 16
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = T.render_accumulate(tree, sc["camera"], lights, conf, acc, sample_offset=1)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        out[name] = {"mrays_s": round(st["rays"] / sec / 1e6, 1),
+                     "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
+                     "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 4)}
+    out["workload"] = ("raycaster and raytracer (max-depth 3, point light, icosphere reflectivity 0.5) on the "
+                       "level-8 icosphere in the Cornell enclosure, 1920x1080, 16 pixel samples, rendering kd-tree")
+    tree.set_reflection(np.zeros((n, 4), np.float32), np.zeros(n, np.float32))
+    return out
 
 
 def cornell_gpu_vs_cpu(args):
