@@ -56,6 +56,9 @@ namespace tn {
 namespace {
 
 constexpr int kRBlock = 256;
+// LDS short-stack entries per lane of the renderer's traversal kernels (16
+// measured no faster on small or large launches than 8)
+constexpr int kRStack = kStack;
 constexpr uint32_t kMaxLevels = 64;
 constexpr uint64_t kSceneSeedStep = 0x9E3779B97F4A7C15ull;
 // scenes with at most this many emitter triangles pick one by solid angle
@@ -315,14 +318,14 @@ struct DynamicChunks {
 template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
     RenderArgs A, Queues Q, int level, int buf) {
-    __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ uint2 stk[kRStack * kRBlock];
     __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[level], Q.cap);  // overflow is reported, never read
     const float4* qo = Q.o[buf];
     const float4* qd = Q.d[buf];
     uint4* hit = Q.hit;
     TraversalCounters cnt;
-    coop_loop<kStack, EXH, false, false, false>(
+    coop_loop<kRStack, EXH, false, false, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 2 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float&, uint32_t& scene) {
@@ -343,11 +346,11 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
 template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
     RenderArgs A, Queues Q, int level) {
-    __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ uint2 stk[kRStack * kRBlock];
     __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     TraversalCounters cnt;
-    coop_loop<kStack, EXH, false, true, false>(
+    coop_loop<kRStack, EXH, false, true, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
@@ -544,12 +547,12 @@ __global__ __launch_bounds__(kRBlock) void k_rt_shade(RenderArgs A, Queues Q, in
 template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_rt_shadow(
     RenderArgs A, Queues Q, int level, int buf) {
-    __shared__ uint2 stk[kStack * kRBlock];
+    __shared__ uint2 stk[kRStack * kRBlock];
     __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     const int ob = buf ^ 1;
     TraversalCounters cnt;
-    coop_loop<kStack, EXH, false, true, false>(
+    coop_loop<kRStack, EXH, false, true, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {

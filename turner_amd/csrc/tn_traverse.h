@@ -233,6 +233,7 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
 template <int S, bool COUNT>
 __device__ __forceinline__ uint2 trav_descend(Trav& T, const SceneView& sc, uint2* stk, int stride,
                                               TraversalCounters& cnt) {
+    static_assert(S > 0 && (S & (S - 1)) == 0, "the circular stack index wraps with & (S - 1)");
     uint32_t node = T.node;
     const float tenter = T.tenter;
     float texit = T.texit;
@@ -272,6 +273,7 @@ __device__ __forceinline__ uint2 trav_descend(Trav& T, const SceneView& sc, uint
 template <int S, bool EXH, bool ANY>
 __device__ __forceinline__ bool trav_after_leaf(Trav& T, float tmax_any, const uint2* stk,
                                                 int stride, TraversalCounters& cnt) {
+    static_assert(S > 0 && (S & (S - 1)) == 0, "the circular stack index wraps with & (S - 1)");
     const float texit = T.texit;
     if (!EXH) {
         if (!ANY && T.best_id != kInvalidId && T.best_r * kExitMargin < texit) return true;
