@@ -191,12 +191,17 @@ def test_cli_raycaster_and_raytracer_match_oracle(gpu, tmp_path):
                                       {"shadow_intensity": 0.4}),
                                      (RAYCASTER, O.RENDERER_RAYCASTER, ["--max-visibility=2000"],
                                       {"max_visibility": 2000.0})):
-        p = run_exe(exe, path, "-w", str(w), "--background", "0.1 0.2 0.3", *extra)
+        pfm = str(tmp_path / (os.path.basename(exe) + ".pfm"))
+        p = run_exe(exe, path, "-w", str(w), "--background", "0.1 0.2 0.3", "--pfm", pfm, *extra)
         assert p.returncode == 0, p.stderr
         head, pix = _ppm_ints(p.stdout)
         assert head == ["P3", f"{w} {w}", "255"] and pix.size == w * w * 3
         lin, _, nr, _ = O.render(sc, ot, w, w, 3, 1, 1, flags=O.FLAG_REF_RNG, bg=(0.1, 0.2, 0.3, 1.0),
                                  nthreads=1, renderer=renderer, **kw)
+        # the linear image (before exposure) as a float map
+        np.testing.assert_allclose(T.read_pfm(pfm), lin[..., :3], rtol=1e-5, atol=1e-6)
+        if renderer == O.RENDERER_RAYCASTER:
+            assert np.array_equal(T.read_pfm(pfm), lin[..., :3])
         want = _display_ppm_ints(lin)
         diff = np.abs(pix - want)
         print(f"[cli {os.path.basename(exe)}] max |d| {diff.max()}, exact {np.mean(diff == 0):.4f}", flush=True)

@@ -24,7 +24,7 @@ from ._lib import (TN_INVALID_ID, TN_RENDERER_PATHTRACER, TN_RENDERER_RAYCASTER,
 
 __all__ = ["KDTree", "KDTreeIntersection", "TracerConfig", "render", "TurnerError",
            "TN_INVALID_ID", "device_info", "build_tree", "tree_cache_save", "tree_cache_load",
-           "write_ppm", "render_accumulate", "TN_RENDERER_PATHTRACER", "TN_RENDERER_RAYCASTER",
+           "write_ppm", "write_pfm", "read_pfm", "render_accumulate", "TN_RENDERER_PATHTRACER", "TN_RENDERER_RAYCASTER",
            "TN_RENDERER_RAYTRACER"]
 
 
@@ -327,6 +327,38 @@ def render_accumulate(tree, cam, lights, conf, accum, sample_offset=0):
                                      C.byref(p), ptr(accum), C.byref(st), stream))
     return {"rays": st.rays, "primary_rays": st.primary_rays, "shadow_rays": st.shadow_rays,
             "ms": st.ms}
+
+
+def write_pfm(path, rgb):
+    """Portable float map of an (h, w, 3|4) image: "PF", width height, -1.0
+    (little-endian), FP32 RGB rows bottom first -- the linear images the
+    relMSE measurements compare."""
+    a = np.asarray(rgb, np.float32)[..., :3]
+    h, w = a.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        f.write(np.ascontiguousarray(a[::-1], "<f4").tobytes())
+
+
+def read_pfm(path):
+    """(h, w, 3) float32 image of a colour PF file (either byte order)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    toks, pos = [], 0
+    while len(toks) < 4:  # PF, width, height, scale: whitespace-separated
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        end = pos
+        while end < len(data) and not data[end:end + 1].isspace():
+            end += 1
+        toks.append(data[pos:end].decode())
+        pos = end
+    pos += 1  # one whitespace byte ends the header
+    if toks[0] != "PF":
+        raise ValueError(f"{path} is not a colour PFM")
+    w, h, scale = int(toks[1]), int(toks[2]), float(toks[3])
+    img = np.frombuffer(data, "<f4" if scale < 0 else ">f4", count=w * h * 3, offset=pos).reshape(h, w, 3)
+    return img[::-1].astype(np.float32)
 
 
 def write_ppm(rgba):

@@ -62,6 +62,7 @@ MI355X options:
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
   --guided                          Learned path guiding (needs -m 1): trains online, then renders.
   --train-passes=<int>              Guiding: training passes of 2 spp [default: 16].
+  --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
 )";
 
@@ -76,6 +77,7 @@ MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per ro
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
   --render-tree                     Rendering kd-tree (same closest hits).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
+  --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
 )";
 
@@ -91,6 +93,7 @@ MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per ro
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
   --render-tree                     Rendering kd-tree (same closest hits).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
+  --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
 )";
 
@@ -177,6 +180,7 @@ int main(int argc, char** argv) {
     using namespace turner_amd;
     TracerConfig conf;
     bool render_tree = false, guided = false;
+    std::string pfm;
     int train_passes = 16;
     try {
         Args a = parse(argc, argv);
@@ -211,6 +215,7 @@ int main(int argc, char** argv) {
         if (a.opt.count("--exhaustive")) conf.flags |= TN_RENDER_EXHAUSTIVE;
         if (a.opt.count("--graph")) conf.flags |= TN_RENDER_GRAPH;
         render_tree = a.opt.count("--render-tree") > 0;
+        pfm = get(a, "--pfm", "");
         guided = kRenderer == TN_RENDERER_PATHTRACER && a.opt.count("--guided") > 0;
         train_passes = std::stoi(get(a, "--train-passes", "16"));
         if (guided && (conf.num_monte_carlo_samples != 1 || train_passes < 0))
@@ -273,6 +278,7 @@ int main(int argc, char** argv) {
                   << "\nLoading time   : " << load_s << " sec\nRendering time : " << ms / 1000.0
                   << " sec" << std::endl;
         std::cout << to_ppm(res.image, conf.width, height) << std::endl;
+        if (!pfm.empty()) write_pfm(pfm, res.linear, conf.width, height);
     } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;
         return 1;

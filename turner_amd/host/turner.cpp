@@ -238,6 +238,24 @@ std::string to_ppm(const std::vector<Color>& image, size_t width, size_t height)
     return s;
 }
 
+void write_pfm(const std::string& path, const std::vector<Color>& image, size_t width, size_t height) {
+    if (image.size() != width * height) throw Error(TN_ERR_INVALID, "image size does not match width x height");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw Error(TN_ERR_INVALID, "cannot write " + path);
+    f << "PF\n" << width << " " << height << "\n-1.0\n";  // negative scale: little-endian
+    std::vector<float> row(3 * width);
+    for (size_t y = height; y-- > 0;) {
+        for (size_t x = 0; x < width; ++x) {
+            const Color& c = image[y * width + x];
+            row[3 * x] = c.r;
+            row[3 * x + 1] = c.g;
+            row[3 * x + 2] = c.b;
+        }
+        f.write(reinterpret_cast<const char*>(row.data()), (std::streamsize)(row.size() * 4));
+    }
+    if (!f) throw Error(TN_ERR_INVALID, "cannot write " + path);
+}
+
 Scene load_tns(const std::string& path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) throw Error(TN_ERR_INVALID, "cannot open scene " + path);

@@ -191,6 +191,9 @@ RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vec
 
 // PPM P3 as lib/raster.h:79-100 writes it.
 std::string to_ppm(const std::vector<Color>& image, size_t width, size_t height);
+// Portable float map (PF, little-endian FP32 RGB, bottom row first) of an
+// image, e.g. the linear mean before exposure, for error measurements.
+void write_pfm(const std::string& path, const std::vector<Color>& image, size_t width, size_t height);
 
 // Scene file written by turner_amd.scenes.save_tns (Python): triangles,
 // vertex normals, RGBA albedo and emission (version 2 adds the reflective

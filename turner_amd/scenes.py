@@ -311,6 +311,122 @@ def reference_furnace(level=3, rho=0.8):
             "camera": camera((0.0, 0.0, -4.0), 40.0), "lights": []}
 
 
+# ------------------------------------------------------------ scene files
+def load_obj(path, albedo=(0.8, 0.8, 0.8), transform=None):
+    """Triangles of a Wavefront OBJ file (with its MTL library) as scene
+    arrays: verts (N, 9), normals (N, 9), albedo (N, 4), emission (N, 4).
+
+    Faces may be polygons (fan-triangulated) with v, v/vt, v//vn or v/vt/vn
+    references, negative (relative) indices allowed.  Corners without a
+    normal take the face normal.  usemtl picks the MTL material: Kd is the
+    diffuse colour (default `albedo`), Ke the emission.  transform (4 x 4)
+    maps vertices, its 3 x 3 part the normals, as the reference applies a
+    node's transformation (main.cpp:33-61).  Camera and lights are the
+    caller's (an OBJ file has none)."""
+    import os
+    base = os.path.dirname(os.path.abspath(path))
+    V, N = [], []
+    mats, cur = {}, None
+    tv, tn, tm = [], [], []
+
+    def load_mtl(fn):
+        name = None
+        with open(os.path.join(base, fn)) as f:
+            for line in f:
+                t = line.split()
+                if not t or t[0].startswith("#"):
+                    continue
+                if t[0] == "newmtl":
+                    name = " ".join(t[1:])
+                    mats[name] = {"Kd": tuple(albedo), "Ke": (0.0, 0.0, 0.0)}
+                elif t[0] in ("Kd", "Ke") and name is not None:
+                    mats[name][t[0]] = tuple(float(x) for x in t[1:4])
+
+    def index(tok, n):
+        k = int(tok)
+        return k - 1 if k > 0 else n + k
+
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t or t[0].startswith("#"):
+                continue
+            if t[0] == "v":
+                V.append([float(x) for x in t[1:4]])
+            elif t[0] == "vn":
+                N.append([float(x) for x in t[1:4]])
+            elif t[0] == "mtllib":
+                load_mtl(" ".join(t[1:]))
+            elif t[0] == "usemtl":
+                cur = " ".join(t[1:])
+            elif t[0] == "f":
+                corners = []
+                for c in t[1:]:
+                    parts = c.split("/")
+                    vi = index(parts[0], len(V))
+                    ni = index(parts[2], len(N)) if len(parts) > 2 and parts[2] else None
+                    corners.append((vi, ni))
+                for k in range(1, len(corners) - 1):
+                    tri = (corners[0], corners[k], corners[k + 1])
+                    tv.append([V[c[0]] for c in tri])
+                    tn.append([N[c[1]] if c[1] is not None else None for c in tri])
+                    tm.append(cur)
+    verts = np.asarray(tv, np.float64).reshape(-1, 3, 3)
+    if transform is not None:
+        T = np.asarray(transform, np.float64)
+        verts = verts @ T[:3, :3].T + T[:3, 3]
+    fn = np.cross(verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0])
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-300)
+    normals = np.empty_like(verts)
+    for i, corner in enumerate(tn):
+        for k in range(3):
+            n = corner[k]
+            if n is None:
+                normals[i, k] = fn[i]
+            else:
+                n = np.asarray(n, np.float64)
+                normals[i, k] = n @ np.asarray(transform, np.float64)[:3, :3].T if transform is not None else n
+    alb = np.array([[*(mats.get(m, {}).get("Kd", albedo)), 1.0] for m in tm], np.float32).reshape(-1, 4)
+    ke = [mats.get(m, {}).get("Ke", (0.0, 0.0, 0.0)) for m in tm]
+    emi = np.array([[*e, 1.0 if any(e) else 0.0] for e in ke], np.float32).reshape(-1, 4)
+    return {"verts": verts.reshape(-1, 9).astype(np.float32), "normals": normals.reshape(-1, 9).astype(np.float32),
+            "albedo": alb, "emission": emi}
+
+
+def save_obj(scene, path):
+    """Writes the triangles of a scene dict as OBJ (three v and vn per
+    triangle, exact float32 text) with an MTL library of its distinct
+    (albedo, emission) pairs next to it; load_obj reads it back."""
+    import os
+    verts = np.asarray(scene["verts"], np.float32).reshape(-1, 3, 3)
+    normals = np.asarray(scene["normals"], np.float32).reshape(-1, 3, 3)
+    alb = np.asarray(scene["albedo"], np.float32).reshape(-1, 4)
+    emi = np.asarray(scene["emission"], np.float32).reshape(-1, 4)
+    mtl_path = os.path.splitext(path)[0] + ".mtl"
+    keys, names = {}, []
+    for a, e in zip(alb, emi):
+        k = (tuple(a[:3].tolist()), tuple(e[:3].tolist()))
+        if k not in keys:
+            keys[k] = f"m{len(keys)}"
+        names.append(keys[k])
+    g = lambda x: repr(float(x))
+    with open(mtl_path, "w") as f:
+        for (kd, ke), name in keys.items():
+            f.write(f"newmtl {name}\nKd {' '.join(map(g, kd))}\nKe {' '.join(map(g, ke))}\n")
+    with open(path, "w") as f:
+        f.write(f"mtllib {os.path.basename(mtl_path)}\n")
+        last = None
+        for i in range(len(verts)):
+            if names[i] != last:
+                f.write(f"usemtl {names[i]}\n")
+                last = names[i]
+            for k in range(3):
+                f.write("v " + " ".join(map(g, verts[i, k])) + "\n")
+            for k in range(3):
+                f.write("vn " + " ".join(map(g, normals[i, k])) + "\n")
+            f.write("f -3//-3 -2//-2 -1//-1\n")
+
+
 def save_tns(scene, path, aspect=1.0):
     """Writes a scene for the C++ front ends (turner_amd/pathtracer,
     raycaster, raytracer): "TNSC", version (1, or 2 when the scene has
