@@ -70,10 +70,12 @@ struct Queues {
     float4* w[2];   // throughput weight (RGBA)
     uint4* rng[2];  // xorshift128 state
     uint4* hit;     // id, r, s, t bits
-    float4* so;     // shadow: origin, pixel (bits)
-    float4* sd;     // shadow: direction, tmax
-    float4* sc;     // shadow: contribution
-    float4* sx;     // shadow: radiance at the segment (training records), slot (bits)
+    // shadow queues, two: level k uses [k & 1], so the shadow test of level
+    // k can run (on a second stream) while level k + 1 is traced and shaded
+    float4* so[2];  // shadow: origin, pixel (bits)
+    float4* sd[2];  // shadow: direction, tmax
+    float4* sc[2];  // shadow: contribution
+    float4* sx[2];  // shadow: radiance at the segment (training records), slot (bits)
     uint32_t cap;   // capacity of each path queue (entries)
     uint32_t scap;  // capacity of the shadow queue
     // device counters: [0..kMaxLevels) path counts, [kMaxLevels..2k) shadow
@@ -349,12 +351,13 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     __shared__ uint2 stk[kRStack * kRBlock];
     __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
+    const int sb = level & 1;
     TraversalCounters cnt;
     coop_loop<kRStack, EXH, false, true, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
-            const float4 a = Q.so[k], b = Q.sd[k];
+            const float4 a = Q.so[sb][k], b = Q.sd[sb][k];
             o = make_float3(a.x, a.y, a.z);
             d = make_float3(b.x, b.y, b.z);
             tmax = b.w;
@@ -362,9 +365,9 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
             if (!traversed || T.best_id == kInvalidId) {
-                accumulate(A.accum, __float_as_uint(Q.so[k].w), Q.sc[k]);
+                accumulate(A.accum, __float_as_uint(Q.so[sb][k].w), Q.sc[sb][k]);
                 if (A.rec_x) {
-                    const float4 x = Q.sx[k];
+                    const float4 x = Q.sx[sb][k];
                     float* dst = reinterpret_cast<float*>(A.rec_x + __float_as_uint(x.w));
                     atomicAdd(dst + 0, x.x);
                     atomicAdd(dst + 1, x.y);
@@ -386,13 +389,13 @@ __device__ __forceinline__ void onb(float3 n, float3& b1, float3& b2) {
     b2 = make_float3(b, sign + n.y * n.y * a, -n.y);
 }
 
-__device__ __forceinline__ void push_shadow(const Queues& Q, uint32_t k, float3 o, uint32_t pixel, float3 dir,
-                                            float tmax, float4 contrib, float4 local, uint32_t slot,
+__device__ __forceinline__ void push_shadow(const Queues& Q, int sb, uint32_t k, float3 o, uint32_t pixel,
+                                            float3 dir, float tmax, float4 contrib, float4 local, uint32_t slot,
                                             bool records) {
-    Q.so[k] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
-    Q.sd[k] = make_float4(dir.x, dir.y, dir.z, tmax);
-    Q.sc[k] = contrib;
-    if (records) Q.sx[k] = make_float4(local.x, local.y, local.z, __uint_as_float(slot));
+    Q.so[sb][k] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
+    Q.sd[sb][k] = make_float4(dir.x, dir.y, dir.z, tmax);
+    Q.sc[sb][k] = contrib;
+    if (records) Q.sx[sb][k] = make_float4(local.x, local.y, local.z, __uint_as_float(slot));
 }
 
 // ------------------------------------------------ raycaster and raytracer
@@ -529,10 +532,11 @@ __global__ __launch_bounds__(kRBlock) void k_rt_shade(RenderArgs A, Queues Q, in
                     // shadowed iff the closest hit is nearer than the light
                     // (strict): any hit with r <= the float below sdist
                     const float tmax = __uint_as_float(__float_as_uint(sdist) - 1u);
-                    Q.so[k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
-                    Q.sd[k] = make_float4(sdir.x, sdir.y, sdir.z, tmax);
-                    Q.sc[k] = contrib;
-                    Q.sx[k] = make_float4(__uint_as_float(child), 0.f, 0.f, 0.f);
+                    const int sb = level & 1;
+                    Q.so[sb][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
+                    Q.sd[sb][k] = make_float4(sdir.x, sdir.y, sdir.z, tmax);
+                    Q.sc[sb][k] = contrib;
+                    Q.sx[sb][k] = make_float4(__uint_as_float(child), 0.f, 0.f, 0.f);
                 } else {
                     atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
                 }
@@ -551,29 +555,30 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     __shared__ unsigned long long slots[kRBlock];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     const int ob = buf ^ 1;
+    const int sb = level & 1;
     TraversalCounters cnt;
     coop_loop<kRStack, EXH, false, true, false>(
         A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
         DynamicChunks{n, Q.cnt + 3 * kMaxLevels + level},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
-            const float4 a = Q.so[k], b = Q.sd[k];
+            const float4 a = Q.so[sb][k], b = Q.sd[sb][k];
             o = make_float3(a.x, a.y, a.z);
             d = make_float3(b.x, b.y, b.z);
             tmax = b.w;
             scene = scene_of_pixel(A, __float_as_uint(a.w));
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
-            float4 c = Q.sc[k];
+            float4 c = Q.sc[sb][k];
             if (traversed && T.best_id != kInvalidId) {
                 const float si = A.shadow_si;
                 c = make_float4(c.x - c.x * si, c.y - c.y * si, c.z - c.z * si, c.w - c.w * si);
-                const uint32_t child = __float_as_uint(Q.sx[k].x);
+                const uint32_t child = __float_as_uint(Q.sx[sb][k].x);
                 if (child != ~0u) {
                     const float4 w = Q.w[ob][child];
                     Q.w[ob][child] = make_float4(w.x - w.x * si, w.y - w.y * si, w.z - w.z * si, w.w - w.w * si);
                 }
             }
-            accumulate(A.accum, __float_as_uint(Q.so[k].w), c);
+            accumulate(A.accum, __float_as_uint(Q.so[sb][k].w), c);
         },
         cnt);
     if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
@@ -747,12 +752,12 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
             sbase = __shfl(sbase, 0);
             if (has_point) {
                 const uint32_t k = sbase + lanes_below(mp);
-                if (k < Q.scap) push_shadow(Q, k, p2, pixel, ldir, ldist, lcontrib, llocal, slot, records);
+                if (k < Q.scap) push_shadow(Q, level & 1, k, p2, pixel, ldir, ldist, lcontrib, llocal, slot, records);
                 else atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
             }
             if (has_area) {
                 const uint32_t k = sbase + (uint32_t)__popcll(mp) + lanes_below(ma);
-                if (k < Q.scap) push_shadow(Q, k, p2, pixel, adir, adist, acontrib, alocal, slot, records);
+                if (k < Q.scap) push_shadow(Q, level & 1, k, p2, pixel, adir, adist, acontrib, alocal, slot, records);
                 else atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
             }
         }
@@ -944,7 +949,22 @@ struct RenderBuffers {
     ChunkState* cs = nullptr;
     SceneParams* sp = nullptr;
     uint32_t sp_cap = 0;
+    // the shadow tests run on a second stream (default priority), ordered
+    // against the main stream by one event per level and kind
+    hipStream_t side = nullptr;
+    hipEvent_t ev_shade[kMaxLevels] = {};
+    hipEvent_t ev_shadow[kMaxLevels] = {};
 };
+
+int ensure_side_stream(RenderBuffers& rb) {
+    if (rb.side) return TN_OK;
+    TN_HIP(hipStreamCreateWithFlags(&rb.side, hipStreamNonBlocking));
+    for (uint32_t k = 0; k < kMaxLevels; ++k) {
+        TN_HIP(hipEventCreateWithFlags(&rb.ev_shade[k], hipEventDisableTiming));
+        TN_HIP(hipEventCreateWithFlags(&rb.ev_shadow[k], hipEventDisableTiming));
+    }
+    return TN_OK;
+}
 
 std::mutex g_render_mutex;
 
@@ -965,10 +985,12 @@ size_t carve(Queues& q, ChunkState** cs, char* base, uint32_t cap) {
     }
     q.hit = (uint4*)take((size_t)cap * 16);
     q.scap = 2 * cap;
-    q.so = (float4*)take((size_t)q.scap * 16);
-    q.sd = (float4*)take((size_t)q.scap * 16);
-    q.sc = (float4*)take((size_t)q.scap * 16);
-    q.sx = (float4*)take((size_t)q.scap * 16);
+    for (int k = 0; k < 2; ++k) {
+        q.so[k] = (float4*)take((size_t)q.scap * 16);
+        q.sd[k] = (float4*)take((size_t)q.scap * 16);
+        q.sc[k] = (float4*)take((size_t)q.scap * 16);
+        q.sx[k] = (float4*)take((size_t)q.scap * 16);
+    }
     q.cnt = (uint32_t*)take((4 * kMaxLevels + 2) * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
     *cs = (ChunkState*)take(sizeof(ChunkState));
@@ -1152,6 +1174,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     std::lock_guard<std::mutex> lock(g_render_mutex);
     static RenderBuffers rb;
     if ((rc = alloc_queues(rb, cap, (uint32_t)table.size()))) return rc;
+    if ((rc = ensure_side_stream(rb))) return rc;
     TN_HIP(hipMemcpyAsync(rb.sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
                           stream));
 
@@ -1214,29 +1237,53 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     const dim3 tgrid((uint32_t)G.cus * bpc), tblock(kRBlock);
     const uint32_t rgrid = (chunk + kRBlock - 1) / kRBlock;
     const Queues Q = rb.q;
-    auto chunk_sequence = [&](hipStream_t s) {
+    // One chunk: raygen, then per level trace and shade on the main stream
+    // s and the level's shadow test on the side stream, so that the shadow
+    // test of level k overlaps the tracing of level k + 1 (each launch's
+    // tail -- its slowest rays -- then leaves room for the other).  Shade of
+    // level k writes shadow queue k & 1, so it waits for the shadow test of
+    // level k - 2; the raytracer's shade also reads the child weights the
+    // shadow test of level k - 1 darkens, so it waits for that one.
+    const hipStream_t side = rb.side;
+    auto chunk_sequence = [&](hipStream_t s) -> hipError_t {
+        hipError_t e = hipSuccess;
+        auto chk = [&](hipError_t x) {
+            if (e == hipSuccess) e = x;
+        };
         k_chunk_begin<<<1, 4 * kMaxLevels + 64, 0, s>>>(Q.cnt, rb.cs);
         if (training) k_rec_clear<<<(uint32_t)(((size_t)chunk * A.rstride + 255) / 256), 256, 0, s>>>(A);
         if (rays) k_raygen_rays<<<rgrid, kRBlock, 0, s>>>(A, Q, ray_o, ray_d);
         else k_raygen<<<rgrid, kRBlock, 0, s>>>(A, Q);
+        const bool shadows = renderer != TN_RENDERER_RAYCASTER;
         int buf = 0;
         for (int lv = 0; lv < levels; ++lv) {
             if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
             else k_trace<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
             if (renderer == TN_RENDERER_RAYCASTER) {
                 k_cast<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            } else if (renderer == TN_RENDERER_RAYTRACER) {
-                k_rt_shade<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-                if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-                else k_rt_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            } else {
-                if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
-                else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-                if (exh) k_shadow<true><<<tgrid, tblock, 0, s>>>(A, Q, lv);
-                else k_shadow<false><<<tgrid, tblock, 0, s>>>(A, Q, lv);
+                buf ^= 1;
+                continue;
             }
+            const int dep = renderer == TN_RENDERER_RAYTRACER ? lv - 1 : lv - 2;
+            if (dep >= 0) chk(hipStreamWaitEvent(s, rb.ev_shadow[dep], 0));
+            if (renderer == TN_RENDERER_RAYTRACER) k_rt_shade<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            else if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
+            else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            chk(hipEventRecord(rb.ev_shade[lv], s));
+            chk(hipStreamWaitEvent(side, rb.ev_shade[lv], 0));
+            if (renderer == TN_RENDERER_RAYTRACER) {
+                if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, side>>>(A, Q, lv, buf);
+                else k_rt_shadow<false><<<tgrid, tblock, 0, side>>>(A, Q, lv, buf);
+            } else {
+                if (exh) k_shadow<true><<<tgrid, tblock, 0, side>>>(A, Q, lv);
+                else k_shadow<false><<<tgrid, tblock, 0, side>>>(A, Q, lv);
+            }
+            chk(hipEventRecord(rb.ev_shadow[lv], side));
             buf ^= 1;
         }
+        // every shadow test is done before the counts are read (the side
+        // stream runs in order, so its last event covers all of them)
+        if (shadows) chk(hipStreamWaitEvent(s, rb.ev_shadow[levels - 1], 0));
         // the raytracer also counts the mirror rays of its last level
         // (trace() is entered and returns {} beyond max_depth)
         k_chunk_totals<<<1, 64, 0, s>>>(Q.cnt, Q.totals,
@@ -1244,6 +1291,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
                                          : renderer == TN_RENDERER_RAYTRACER ? levels + 1
                                                                              : levels);
         if (training) k_collect<<<rgrid, kRBlock, 0, s>>>(A, P->max_depth);
+        return e;
     };
     const uint64_t nchunks = (total + chunk - 1) / chunk;
     auto t0 = std::chrono::steady_clock::now();
@@ -1254,8 +1302,9 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         TN_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            chunk_sequence(cap_stream);
+            const hipError_t es = chunk_sequence(cap_stream);  // the side stream joins and leaves the capture
             e = hipStreamEndCapture(cap_stream, &graph);
+            if (e == hipSuccess) e = es;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         for (uint64_t c = 0; e == hipSuccess && c < nchunks; ++c) e = hipGraphLaunch(exec, stream);
@@ -1268,7 +1317,13 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
             return hip_error(e, "captured chunk sequence");
         }
     } else {
-        for (uint64_t c = 0; c < nchunks; ++c) chunk_sequence(stream);
+        for (uint64_t c = 0; c < nchunks; ++c) {
+            const hipError_t e = chunk_sequence(stream);
+            if (e != hipSuccess) {
+                if (d_jit) (void)hipFree(d_jit);
+                return hip_error(e, "chunk sequence");
+            }
+        }
     }
     TN_HIP(hipGetLastError());
     if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return rc;
