@@ -310,8 +310,9 @@ int tn_guide_train_step(tn_guide* g, const float* p3, const float* n3, const flo
  * and the loss of the last step. */
 int tn_guide_info(tn_guide* g, uint32_t* train_samples, uint32_t* steps, float* last_loss);
 /* Copies up to cap samples of that training set to host memory, 12 floats
- * each: p.xyz, c, n.xyz, cos/pdf, w.xyz, tag (uint32 bits: path index within
- * its chunk << 6 | bounce level).  Returns the number copied. */
+ * each: p.xyz, c, n.xyz, cos/pdf, w.xyz, tag (uint32 bits: the path's sample
+ * index within its render call, modulo 2^26, << 6 | bounce level).  Returns
+ * the number copied. */
 long tn_guide_get_train_set(tn_guide* g, float* out, size_t cap);
 
 /*

@@ -63,6 +63,8 @@ constexpr uint32_t kMaxLevels = 64;
 constexpr uint64_t kSceneSeedStep = 0x9E3779B97F4A7C15ull;
 // scenes with at most this many emitter triangles pick one by solid angle
 constexpr uint32_t kSolidAngleSelect = 4;
+// concurrent render pipelines (queues + streams) that take alternate chunks
+constexpr int kPipes = 2;
 
 struct Queues {
     float4* o[2];   // o.xyz, pixel (bits)
@@ -104,6 +106,7 @@ struct SceneParams {
 // serves every chunk: k_chunk_begin advances them.
 struct ChunkState {
     unsigned long long next, total, first;
+    unsigned long long stride;  // samples from one chunk of this pipeline to its next
     uint32_t chunk, count;
 };
 
@@ -215,18 +218,20 @@ __global__ void k_chunk_begin(uint32_t* cnt, ChunkState* cs) {
     const unsigned long long first = cs->next;
     const unsigned long long left = cs->total > first ? cs->total - first : 0ull;
     const uint32_t count = (uint32_t)(left < cs->chunk ? left : cs->chunk);
-    if (i < 4 * kMaxLevels + 2) cnt[i] = (i == 0) ? count : 0u;
+    // the overflow word [4 kMaxLevels] lives for the whole render
+    if (i < 4 * kMaxLevels || i == 4 * kMaxLevels + 1) cnt[i] = (i == 0) ? count : 0u;
     __syncthreads();
     if (i == 0) {
         cs->first = first;
         cs->count = count;
-        cs->next = first + count;
+        cs->next = first + cs->stride;
     }
 }
 
-__global__ void k_zero_totals(unsigned long long* totals, uint32_t* incomplete) {
+__global__ void k_zero_totals(unsigned long long* totals, uint32_t* cnt, uint32_t* incomplete) {
     if (threadIdx.x < 2) totals[threadIdx.x] = 0;
-    if (threadIdx.x == 2) *incomplete = 0;
+    if (threadIdx.x == 2) cnt[4 * kMaxLevels] = 0;  // queue overflow
+    if (threadIdx.x == 3 && incomplete) *incomplete = 0;
 }
 
 // levels < 0: the raycaster, whose rays are its hits (cnt[4 kMaxLevels + 1])
@@ -873,7 +878,8 @@ __global__ void k_rec_clear(RenderArgs A) {
 // Walks each path of the chunk backwards (L_k = X_{k+1} + g_{k+1} L_{k+1})
 // and appends (p, n, w, lum(L_k) cos / pdf) of every bounce that brought
 // light back to the training set (kept with probability A.accept).  Sample
-// layout: (p, c), (n, cos / pdf), (w, path << 6 | level as bits).
+// layout: (p, c), (n, cos / pdf), (w, sample << 6 | level as bits), sample =
+// the path's sample index within the call modulo 2^26.
 __global__ __launch_bounds__(kRBlock) void k_collect(RenderArgs A, int levels_with_records) {
     const uint32_t npaths = A.cs->count;
     const uint64_t salt = A.seed_pass ^ (A.cs->first * 0x9E3779B97F4A7C15ull);
@@ -922,7 +928,8 @@ __global__ __launch_bounds__(kRBlock) void k_collect(RenderArgs A, int levels_wi
         if (t < A.ts_cap) {
             A.ts[3 * (size_t)t] = make_float4(rp.x, rp.y, rp.z, c);
             A.ts[3 * (size_t)t + 1] = make_float4(rn.x, rn.y, rn.z, rp.w);
-            A.ts[3 * (size_t)t + 2] = make_float4(rw.x, rw.y, rw.z, __uint_as_float((pid << 6) | (uint32_t)k));
+            const uint32_t sample = (uint32_t)(A.cs->first + pid) & ((1u << 26) - 1u);
+            A.ts[3 * (size_t)t + 2] = make_float4(rw.x, rw.y, rw.z, __uint_as_float((sample << 6) | (uint32_t)k));
         }
     }
 }
@@ -1035,11 +1042,13 @@ int alloc_records(RecordBuffers& rb, size_t slots) {
     return TN_OK;
 }
 
-__global__ void k_chunk_setup(ChunkState* cs, unsigned long long total, uint32_t chunk) {
+// Pipeline p of n renders chunks p, p + n, p + 2n, ... of the call's samples.
+__global__ void k_chunk_setup(ChunkState* cs, unsigned long long total, uint32_t chunk, uint32_t p, uint32_t n) {
     if (threadIdx.x == 0) {
-        cs->next = 0;
+        cs->next = (unsigned long long)p * chunk;
         cs->total = total;
         cs->first = 0;
+        cs->stride = (unsigned long long)n * chunk;
         cs->chunk = chunk;
         cs->count = 0;
     }
@@ -1168,115 +1177,151 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     if (renderer != TN_RENDERER_PATHTRACER) q.mc_samples = 1;
     uint32_t chunk = 0, cap = 0;
     if ((rc = chunk_size(&q, shape.chunk, chunk, cap))) return rc;
+    // Two pipelines, each with its own queues and streams, render alternate
+    // chunks concurrently, so that one's launch tails (the slowest rays of a
+    // level) overlap the other's work.  A render of one chunk is split evenly
+    // between them when each half keeps at least 2^19 paths (smaller halves
+    // measured slower than one pipeline).  A graph-replayed render uses one
+    // pipeline: capturing the second pipeline's stream, which itself forks
+    // its shadow stream, crashed the host process, so that combination
+    // stays off.
+    int npipes = shape.graph ? 1 : kPipes;
+    if (npipes > 1 && total <= chunk) {
+        if (total >= (uint64_t)npipes << 19) {
+            const uint64_t part = ((total + npipes - 1) / npipes + 255) & ~255ull;
+            if ((rc = chunk_size(&q, (uint32_t)part, chunk, cap))) return rc;
+        } else {
+            npipes = 1;
+        }
+    }
     if (st) *st = tn_render_stats{};
     if (total == 0) return TN_OK;
 
     std::lock_guard<std::mutex> lock(g_render_mutex);
-    static RenderBuffers rb;
-    if ((rc = alloc_queues(rb, cap, (uint32_t)table.size()))) return rc;
-    if ((rc = ensure_side_stream(rb))) return rc;
-    TN_HIP(hipMemcpyAsync(rb.sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
+    static RenderBuffers rbs[kPipes];
+    static hipStream_t pipe_stream = nullptr;  // main stream of pipeline 1 (pipeline 0 uses the caller's)
+    static hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    for (int p = 0; p < npipes; ++p) {
+        if ((rc = alloc_queues(rbs[p], cap, (uint32_t)table.size()))) return rc;
+        if ((rc = ensure_side_stream(rbs[p]))) return rc;
+    }
+    if (!pipe_stream) {
+        TN_HIP(hipStreamCreateWithFlags(&pipe_stream, hipStreamNonBlocking));
+        TN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        TN_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    TN_HIP(hipMemcpyAsync(rbs[0].sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
                           stream));
 
-    RenderArgs A{};
-    A.sv = G.sv;
-    A.normals = G.normals;
-    A.albedo = G.albedo;
-    A.emission = G.emission;
-    A.trirec = G.trirec;
-    A.emit_ids = G.emit_ids;
-    A.emit_cdf = G.emit_cdf;
-    A.sp = rb.sp;
-    A.nscenes = (uint32_t)table.size();
-    A.cs = rb.cs;
-    A.status = G.status;
-    A.max_depth = q.max_depth;
-    A.mc = P->mc_samples;
-    A.max_vis = P->max_visibility;
-    A.shadow_si = P->shadow_intensity;
-    A.reflective = G.reflective;
-    A.reflectivity = G.reflectivity;
-    A.flags = P->flags;
-    A.seed = P->seed;
-    A.accum = (float4*)accum;
-    A.seed_pass = P->seed + (uint64_t)P->sample_offset * 0xD1B54A32D192ED03ull;
+    RenderArgs A0{};
+    A0.sv = G.sv;
+    A0.normals = G.normals;
+    A0.albedo = G.albedo;
+    A0.emission = G.emission;
+    A0.trirec = G.trirec;
+    A0.emit_ids = G.emit_ids;
+    A0.emit_cdf = G.emit_cdf;
+    A0.sp = rbs[0].sp;
+    A0.nscenes = (uint32_t)table.size();
+    A0.status = G.status;
+    A0.max_depth = q.max_depth;
+    A0.mc = P->mc_samples;
+    A0.max_vis = P->max_visibility;
+    A0.shadow_si = P->shadow_intensity;
+    A0.reflective = G.reflective;
+    A0.reflectivity = G.reflectivity;
+    A0.flags = P->flags;
+    A0.seed = P->seed;
+    A0.accum = (float4*)accum;
+    A0.seed_pass = P->seed + (uint64_t)P->sample_offset * 0xD1B54A32D192ED03ull;
     if (guided) {
-        A.gv = guide->view();
-        A.mis = guide->mis_guided;
+        A0.gv = guide->view();
+        A0.mis = guide->mis_guided;
     }
-    static RecordBuffers recs;
     if (training) {
-        A.rstride = (uint32_t)P->max_depth + 1;
-        if ((rc = alloc_records(recs, (size_t)chunk * A.rstride))) return rc;
-        const size_t ns = recs.slots;
-        A.rec_p = recs.base;
-        A.rec_n = recs.base + ns;
-        A.rec_w = recs.base + 2 * ns;
-        A.rec_g = recs.base + 3 * ns;
-        A.rec_x = recs.base + 4 * ns;
+        A0.rstride = (uint32_t)P->max_depth + 1;
         // every bounce of every path is a candidate; keep about kTrainSet of them
         constexpr uint32_t kTrainSet = 1u << 21;
         if ((rc = guide_reserve_set(guide, kTrainSet, stream))) return rc;
         const double candidates = (double)total * std::max(1, P->max_depth);
-        A.accept = (float)std::min(1.0, 0.95 * kTrainSet / std::max(1.0, candidates));
-        A.ts = guide->ts;
-        A.ts_count = guide->ts_count;
-        A.ts_cap = guide->ts_cap;
+        A0.accept = (float)std::min(1.0, 0.95 * kTrainSet / std::max(1.0, candidates));
+        A0.ts = guide->ts;
+        A0.ts_count = guide->ts_count;
+        A0.ts_cap = guide->ts_cap;
     }
     float* d_jit = nullptr;
     if (jitter_host) {
         TN_HIP(hipMalloc(&d_jit, jitter_len * sizeof(float)));
         TN_HIP(hipMemcpyAsync(d_jit, jitter_host, jitter_len * sizeof(float), hipMemcpyHostToDevice, stream));
-        A.jitter = d_jit;
+        A0.jitter = d_jit;
     }
-    k_zero_totals<<<1, 64, 0, stream>>>(rb.q.totals, G.status + 2);
-    k_chunk_setup<<<1, 64, 0, stream>>>(rb.cs, total, chunk);
+    static RecordBuffers recs[kPipes];
+    RenderArgs A[kPipes];
+    Queues Q[kPipes];
+    for (int p = 0; p < npipes; ++p) {
+        A[p] = A0;
+        A[p].cs = rbs[p].cs;
+        Q[p] = rbs[p].q;
+        if (training) {
+            if ((rc = alloc_records(recs[p], (size_t)chunk * A0.rstride))) return rc;
+            const size_t ns = recs[p].slots;
+            A[p].rec_p = recs[p].base;
+            A[p].rec_n = recs[p].base + ns;
+            A[p].rec_w = recs[p].base + 2 * ns;
+            A[p].rec_g = recs[p].base + 3 * ns;
+            A[p].rec_x = recs[p].base + 4 * ns;
+        }
+        k_zero_totals<<<1, 64, 0, stream>>>(Q[p].totals, Q[p].cnt, p == 0 ? G.status + 2 : nullptr);
+        k_chunk_setup<<<1, 64, 0, stream>>>(rbs[p].cs, total, chunk, (uint32_t)p, (uint32_t)npipes);
+    }
     const bool exh = P->flags & TN_RENDER_EXHAUSTIVE;
     const int levels = q.max_depth + 1;
     const uint32_t bpc = std::max(1u, shape.blocks_per_cu);
     const dim3 tgrid((uint32_t)G.cus * bpc), tblock(kRBlock);
     const uint32_t rgrid = (chunk + kRBlock - 1) / kRBlock;
-    const Queues Q = rb.q;
-    // One chunk: raygen, then per level trace and shade on the main stream
-    // s and the level's shadow test on the side stream, so that the shadow
-    // test of level k overlaps the tracing of level k + 1 (each launch's
-    // tail -- its slowest rays -- then leaves room for the other).  Shade of
-    // level k writes shadow queue k & 1, so it waits for the shadow test of
-    // level k - 2; the raytracer's shade also reads the child weights the
-    // shadow test of level k - 1 darkens, so it waits for that one.
-    const hipStream_t side = rb.side;
-    auto chunk_sequence = [&](hipStream_t s) -> hipError_t {
+    // One chunk of pipeline p: raygen, then per level trace and shade on the
+    // main stream s and the level's shadow test on the pipeline's side
+    // stream, so that the shadow test of level k overlaps the tracing of
+    // level k + 1.  Shade of level k writes shadow queue k & 1, so it waits
+    // for the shadow test of level k - 2; the raytracer's shade also reads
+    // the child weights the shadow test of level k - 1 darkens, so it waits
+    // for that one.
+    auto chunk_sequence = [&](int p, hipStream_t s) -> hipError_t {
         hipError_t e = hipSuccess;
         auto chk = [&](hipError_t x) {
             if (e == hipSuccess) e = x;
         };
-        k_chunk_begin<<<1, 4 * kMaxLevels + 64, 0, s>>>(Q.cnt, rb.cs);
-        if (training) k_rec_clear<<<(uint32_t)(((size_t)chunk * A.rstride + 255) / 256), 256, 0, s>>>(A);
-        if (rays) k_raygen_rays<<<rgrid, kRBlock, 0, s>>>(A, Q, ray_o, ray_d);
-        else k_raygen<<<rgrid, kRBlock, 0, s>>>(A, Q);
+        RenderBuffers& rb = rbs[p];
+        const RenderArgs& Ap = A[p];
+        const Queues& Qp = Q[p];
+        const hipStream_t side = rb.side;
+        k_chunk_begin<<<1, 4 * kMaxLevels + 64, 0, s>>>(Qp.cnt, rb.cs);
+        if (training) k_rec_clear<<<(uint32_t)(((size_t)chunk * Ap.rstride + 255) / 256), 256, 0, s>>>(Ap);
+        if (rays) k_raygen_rays<<<rgrid, kRBlock, 0, s>>>(Ap, Qp, ray_o, ray_d);
+        else k_raygen<<<rgrid, kRBlock, 0, s>>>(Ap, Qp);
         const bool shadows = renderer != TN_RENDERER_RAYCASTER;
         int buf = 0;
         for (int lv = 0; lv < levels; ++lv) {
-            if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            else k_trace<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+            else k_trace<false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
             if (renderer == TN_RENDERER_RAYCASTER) {
-                k_cast<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+                k_cast<<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
                 buf ^= 1;
                 continue;
             }
             const int dep = renderer == TN_RENDERER_RAYTRACER ? lv - 1 : lv - 2;
             if (dep >= 0) chk(hipStreamWaitEvent(s, rb.ev_shadow[dep], 0));
-            if (renderer == TN_RENDERER_RAYTRACER) k_rt_shade<<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
-            else if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(A, Q, lv, buf);
-            else k_shade<false><<<tgrid, tblock, 0, s>>>(A, Q, lv, buf);
+            if (renderer == TN_RENDERER_RAYTRACER) k_rt_shade<<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+            else if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(Ap, Qp, lv, buf);
+            else k_shade<false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
             chk(hipEventRecord(rb.ev_shade[lv], s));
             chk(hipStreamWaitEvent(side, rb.ev_shade[lv], 0));
             if (renderer == TN_RENDERER_RAYTRACER) {
-                if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, side>>>(A, Q, lv, buf);
-                else k_rt_shadow<false><<<tgrid, tblock, 0, side>>>(A, Q, lv, buf);
+                if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv, buf);
+                else k_rt_shadow<false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv, buf);
             } else {
-                if (exh) k_shadow<true><<<tgrid, tblock, 0, side>>>(A, Q, lv);
-                else k_shadow<false><<<tgrid, tblock, 0, side>>>(A, Q, lv);
+                if (exh) k_shadow<true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
+                else k_shadow<false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
             }
             chk(hipEventRecord(rb.ev_shadow[lv], side));
             buf ^= 1;
@@ -1286,28 +1331,42 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         if (shadows) chk(hipStreamWaitEvent(s, rb.ev_shadow[levels - 1], 0));
         // the raytracer also counts the mirror rays of its last level
         // (trace() is entered and returns {} beyond max_depth)
-        k_chunk_totals<<<1, 64, 0, s>>>(Q.cnt, Q.totals,
-                                         renderer == TN_RENDERER_RAYCASTER   ? -1
-                                         : renderer == TN_RENDERER_RAYTRACER ? levels + 1
-                                                                             : levels);
-        if (training) k_collect<<<rgrid, kRBlock, 0, s>>>(A, P->max_depth);
+        k_chunk_totals<<<1, 64, 0, s>>>(Qp.cnt, Qp.totals,
+                                          renderer == TN_RENDERER_RAYCASTER   ? -1
+                                          : renderer == TN_RENDERER_RAYTRACER ? levels + 1
+                                                                              : levels);
+        if (training) k_collect<<<rgrid, kRBlock, 0, s>>>(Ap, P->max_depth);
+        return e;
+    };
+    // One round: pipeline 0 on s0, pipeline 1 forked onto its own stream and
+    // joined back (the fork and join are events, so a captured round is a
+    // graph with the same concurrency).
+    auto round = [&](hipStream_t s0) -> hipError_t {
+        if (npipes == 1) return chunk_sequence(0, s0);
+        hipError_t e = hipEventRecord(ev_fork, s0);
+        if (e == hipSuccess) e = hipStreamWaitEvent(pipe_stream, ev_fork, 0);
+        if (e == hipSuccess) e = chunk_sequence(0, s0);
+        if (e == hipSuccess) e = chunk_sequence(1, pipe_stream);
+        if (e == hipSuccess) e = hipEventRecord(ev_join, pipe_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, ev_join, 0);
         return e;
     };
     const uint64_t nchunks = (total + chunk - 1) / chunk;
+    const uint64_t rounds = (nchunks + npipes - 1) / npipes;
     auto t0 = std::chrono::steady_clock::now();
-    if (shape.graph && nchunks > 1) {
+    if (shape.graph && rounds > 1) {
         hipStream_t cap_stream = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         TN_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            const hipError_t es = chunk_sequence(cap_stream);  // the side stream joins and leaves the capture
+            const hipError_t es = round(cap_stream);  // the other streams join and leave the capture
             e = hipStreamEndCapture(cap_stream, &graph);
             if (e == hipSuccess) e = es;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        for (uint64_t c = 0; e == hipSuccess && c < nchunks; ++c) e = hipGraphLaunch(exec, stream);
+        for (uint64_t c = 0; e == hipSuccess && c < rounds; ++c) e = hipGraphLaunch(exec, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1317,8 +1376,8 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
             return hip_error(e, "captured chunk sequence");
         }
     } else {
-        for (uint64_t c = 0; c < nchunks; ++c) {
-            const hipError_t e = chunk_sequence(stream);
+        for (uint64_t c = 0; c < rounds; ++c) {
+            const hipError_t e = round(stream);
             if (e != hipSuccess) {
                 if (d_jit) (void)hipFree(d_jit);
                 return hip_error(e, "chunk sequence");
@@ -1327,21 +1386,24 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     }
     TN_HIP(hipGetLastError());
     if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return rc;
-    unsigned long long tot[2] = {0, 0};
-    uint32_t overflow = 0, incomplete = 0;
-    TN_HIP(hipMemcpyAsync(tot, Q.totals, 16, hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipMemcpyAsync(&overflow, Q.cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
+    unsigned long long tot[kPipes][2] = {};
+    uint32_t overflow[kPipes] = {}, incomplete = 0;
+    for (int p = 0; p < npipes; ++p) {
+        TN_HIP(hipMemcpyAsync(tot[p], Q[p].totals, 16, hipMemcpyDeviceToHost, stream));
+        TN_HIP(hipMemcpyAsync(&overflow[p], Q[p].cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
+    }
     TN_HIP(hipMemcpyAsync(&incomplete, G.status + 2, 4, hipMemcpyDeviceToHost, stream));
     TN_HIP(hipStreamSynchronize(stream));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (d_jit) (void)hipFree(d_jit);
-    if (overflow) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
+    for (int p = 0; p < npipes; ++p)
+        if (overflow[p]) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
     if (incomplete) return set_error(TN_ERR_INCOMPLETE, "%u render traversals hit the visit cap", incomplete);
     if (st) {
-        st->rays = tot[0];
+        st->rays = tot[0][0] + tot[1][0];
         st->primary_rays = rays ? 0 : total;
-        st->shadow_rays = tot[1];
+        st->shadow_rays = tot[0][1] + tot[1][1];
         st->ms = ms;
     }
     return TN_OK;

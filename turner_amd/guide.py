@@ -126,7 +126,8 @@ class Guide:
 
     def train_set(self, cap=1 << 22):
         """The training set of the last training render as a dict of host
-        arrays: p, c, n, cos_pdf, w, path, level."""
+        arrays: p, c, n, cos_pdf, w, path (the sample index within the render
+        call, modulo 2^26), level."""
         n = self.info()["train_samples"]
         n = min(n, cap)
         buf = np.zeros((max(n, 1), 12), np.float32)
