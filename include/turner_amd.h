@@ -184,7 +184,7 @@ typedef struct {
 #define TN_RENDER_GUIDED 8u      /* learned guiding proposal (needs a tn_guide) */
 #define TN_RENDER_EXHAUSTIVE 16u /* reference traversal order without early exit */
 #define TN_RENDER_GRAPH 32u      /* capture the per-chunk launch sequence once in a hipGraph, replay per chunk */
-#define TN_RENDER_AUTOTUNE 64u   /* batches: chunk size and grid picked by timing probes (cached) */
+#define TN_RENDER_AUTOTUNE 64u   /* batches: chunk size, grid and graph replay picked by timing probes (cached) */
 
 typedef struct {
     uint32_t width, height;

@@ -1548,20 +1548,24 @@ int autotune(tn_batch* b, const std::vector<SceneParams>& table, uint64_t total,
         if (chunk_size(P, c, ch, cap) != TN_OK || cap > (1u << 24)) continue;  // queue budget
         if (c > 2 * probe && c != chunks[0]) continue;
         for (uint32_t g : grids) {
-            LaunchShape sh;
-            sh.chunk = c;
-            sh.blocks_per_cu = g;
-            sh.graph = true;
-            tn_render_stats st{};
-            rc = render_core(b->geom(), table, probe, P, sh, scratch, &st, stream, nullptr, nullptr, nullptr, 0,
-                             nullptr, 0);
-            if (rc) break;
-            const double rate = (double)probe / std::max(st.ms, 1e-3);
-            ++info.candidates;
-            if (rate > best_rate) {
-                best_rate = rate;
-                best = sh;
+            // graph replay (one pipeline) or direct launches (two pipelines)
+            for (const bool graph : {true, false}) {
+                LaunchShape sh;
+                sh.chunk = c;
+                sh.blocks_per_cu = g;
+                sh.graph = graph;
+                tn_render_stats st{};
+                rc = render_core(b->geom(), table, probe, P, sh, scratch, &st, stream, nullptr, nullptr, nullptr,
+                                 0, nullptr, 0);
+                if (rc) break;
+                const double rate = (double)probe / std::max(st.ms, 1e-3);
+                ++info.candidates;
+                if (rate > best_rate) {
+                    best_rate = rate;
+                    best = sh;
+                }
             }
+            if (rc) break;
         }
         if (rc) break;
     }
