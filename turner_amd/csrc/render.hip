@@ -1534,7 +1534,9 @@ int batch_table(const tn_batch* b, const tn_batch_job* jobs, std::vector<ScenePa
 // scratch accumulator) and returns the fastest.
 int autotune(tn_batch* b, const std::vector<SceneParams>& table, uint64_t total, uint64_t pixels,
              const tn_render_params* P, hipStream_t stream, LaunchShape& best, tn_batch_tuning& info) {
-    const uint64_t probe = std::min<uint64_t>(total, 1ull << 21);
+    // the probe spans at least two of the largest chunks, so both pipelines
+    // of a direct-launch shape are busy as in the full render
+    const uint64_t probe = std::min<uint64_t>(total, 1ull << 23);
     float* scratch = nullptr;
     TN_HIP(hipMalloc(&scratch, (size_t)pixels * 16));
     const uint32_t chunks[] = {1u << 18, 1u << 20, 1u << 22};
