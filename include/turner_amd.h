@@ -309,6 +309,11 @@ int tn_guide_train_step(tn_guide* g, const float* p3, const float* n3, const flo
 /* Samples in the training set of the last training render, Adam steps taken
  * and the loss of the last step. */
 int tn_guide_info(tn_guide* g, uint32_t* train_samples, uint32_t* steps, float* last_loss);
+/* One more training pass on the set the last training render recorded: a new
+ * uniform subset of train_samples records, epochs * train_samples / batch
+ * Adam steps (as tn_render_guided trains after rendering).  Lets one render
+ * supply the data of several training passes.  Asynchronous on `stream`. */
+int tn_guide_train_recorded(tn_guide* g, uint32_t train_samples, void* stream);
 /* Copies up to cap samples of that training set to host memory, 12 floats
  * each: p.xyz, c, n.xyz, cos/pdf, w.xyz, tag (uint32 bits: the path's sample
  * index within its render call, modulo 2^26, << 6 | bounce level).  Returns

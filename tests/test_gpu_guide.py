@@ -454,4 +454,31 @@ def test_guide_errors(gpu):
         render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1024)
     render_guided(tree, sc["camera"], sc["lights"], conf, g, acc)  # inference-only guiding works with m > 1
     assert torch.isfinite(acc).all()
+    with pytest.raises(T.TurnerError):  # nothing recorded yet
+        g.train_recorded(1 << 10)
+    g.close()
+
+
+def test_train_recorded_takes_more_passes(gpu):
+    """tn_guide_train_recorded trains another pass (its own subset, the
+    configured number of steps) on the records of the last training render."""
+    torch = _torch()
+    from turner_amd.guide import Guide, render_guided
+    sc = scenes.sphere_in_cornell(level=3)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    g = Guide.for_tree(tree, seed=3, batch=4096, epochs=2)
+    conf = T.TracerConfig(width=32, max_recursion_depth=3, num_pixel_samples=4, num_monte_carlo_samples=1,
+                          flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS)
+    acc = torch.zeros((32, 32, 4), dtype=torch.float32, device="cuda")
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1 << 12)
+    info = g.info()
+    assert info["steps"] == 2 and info["train_samples"] > 0  # 2 epochs x 4096 / 4096
+    theta = g.params()
+    with pytest.raises(T.TurnerError):
+        g.train_recorded(0)
+    g.train_recorded(1 << 13)
+    torch.cuda.synchronize()
+    after = g.info()
+    assert after["steps"] == info["steps"] + 4 and after["train_samples"] == info["train_samples"]
+    assert np.isfinite(after["loss"]) and not np.array_equal(g.params(), theta)
     g.close()

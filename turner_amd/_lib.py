@@ -134,6 +134,7 @@ SIGNATURES = {
     "tn_guide_train_step": (C.c_int, [_P, _P, _P, _P, _P, _U32, _FP, _P]),
     "tn_guide_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), _FP]),
     "tn_guide_get_train_set": (C.c_long, [_P, _P, C.c_size_t]),
+    "tn_guide_train_recorded": (C.c_int, [_P, _U32, _P]),
     "tn_render_guided": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(Light), _U32,
                                    C.POINTER(RenderParams), _P, _U32, _P,
                                    C.POINTER(RenderStats), _P]),

@@ -724,6 +724,14 @@ int tn_guide_loss_grad(tn_guide* g, const float* p3, const float* n3, const floa
     return TN_OK;
 }
 
+int tn_guide_train_recorded(tn_guide* g, uint32_t train_samples, void* stream) {
+    int rc = check_guide(g);
+    if (rc) return rc;
+    if (train_samples == 0) return set_error(TN_ERR_INVALID, "train_samples must be positive");
+    if (!g->ts) return set_error(TN_ERR_INVALID, "no recorded training set: render with train_samples > 0 first");
+    return guide_train_from_set(g, train_samples, (hipStream_t)stream);
+}
+
 int tn_guide_train_step(tn_guide* g, const float* p3, const float* n3, const float* w3, const float* c,
                         uint32_t n, float* loss, void* stream) {
     int rc = check_guide(g);

@@ -119,6 +119,13 @@ class Guide:
                                         C.byref(loss) if want_loss else None, _stream(p)))
         return loss.value if want_loss else None
 
+    def train_recorded(self, train_samples=1 << 18, stream=None):
+        """One more training pass on the records of the last training render
+        (a new uniform subset of train_samples of them)."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        check(lib().tn_guide_train_recorded(self.handle, train_samples, s))
+
     def info(self):
         ns, st, loss = C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().tn_guide_info(self.handle, C.byref(ns), C.byref(st), C.byref(loss)))

@@ -26,7 +26,8 @@ def _conf(width, height, depth, flags, spp, seed):
 
 def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=64,
                    train_passes=16, train_samples=1 << 18, train_spp=2, max_seconds=60.0, seed=1,
-                   max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0):
+                   max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0,
+                   passes_per_render=1):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
     reference's per-pixel variance (relmse subtracts it).
@@ -35,7 +36,10 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
     in advance, so the image stays unbiased); their time still counts.
     train_every=k > 0 keeps training after that phase on every k-th image
     pass (those passes are rendered before the update and stay in the
-    image)."""
+    image).  passes_per_render=p renders the data of p training passes in
+    one render of p * train_spp samples per pixel and trains p passes on it
+    (each on its own uniform subset of train_samples records); a small render
+    costs about as much as a larger one, its slowest rays bound it."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -44,11 +48,14 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
     spp_img = 0  # samples in acc
     err = float("inf")
     k = 0
+    ppr = max(1, int(passes_per_render))
+    train_renders = -(-train_passes // ppr) if train_passes > 0 else 0
     while spent < max_seconds and spp < max_spp:
-        training = guide is not None and k < train_passes
+        training = guide is not None and k < train_renders
         late = (guide is not None and not training and train_every > 0
-                and (k - train_passes) % train_every == 0)
-        n = train_spp if training else spp_per_pass
+                and (k - train_renders) % train_every == 0)
+        extra = min(ppr, train_passes - k * ppr) - 1 if training else 0  # more passes on this render's data
+        n = train_spp * (extra + 1) if training else spp_per_pass
         conf = _conf(w, h, depth, flags, n, seed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -57,6 +64,8 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
         else:
             st = render_guided(tree, scene["camera"], scene["lights"], conf, guide, tacc if training else acc,
                                train_samples=train_samples if (training or late) else 0, sample_offset=spp)
+            for _ in range(extra):
+                guide.train_recorded(train_samples)
         torch.cuda.synchronize()
         spent += time.perf_counter() - t0
         spp += n
