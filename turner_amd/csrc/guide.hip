@@ -118,17 +118,9 @@ __global__ __launch_bounds__(64) void k_guide_forward(GuideView G, const float* 
                                                       const float* __restrict__ n3, uint32_t n,
                                                       float* __restrict__ out) {
     __shared__ __align__(16) uint16_t lds[64 * (kGStrideIn + kGStrideHid)];
-    // K = 80 = 2 x 32 + 16 for layer 1, 64 = 2 x 32 for the others
-    Frags32<2, true, 4> w1;
-    Frags32<2, false, 4> w2, w3;
-    Frags32<2, false, 3> w4;
-    load_frags32(G.wbf + kGW1, w1);
-    load_frags32(G.wbf + kGW2, w2);
-    load_frags32(G.wbf + kGW3, w3);
-    load_frags32(G.wbf + kGW4, w4);
+    GuideFrags F;  // register-resident for every group this wave takes
+    load_guide_frags(G, F);
     const uint32_t l = threadIdx.x;
-    uint16_t* a = lds;
-    uint16_t* b = lds + 64 * kGStrideIn;
     float* o = reinterpret_cast<float*>(lds);
     const uint32_t groups = (n + 63) / 64;
     for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
@@ -139,16 +131,7 @@ __global__ __launch_bounds__(64) void k_guide_forward(GuideView G, const float* 
             p = load3(p3, i);
             nn = load3(n3, i);
         }
-        guide_features(G, p, nn, act, a + l * kGStrideIn);
-        __syncthreads();
-        mfma_wx32(w1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
-        __syncthreads();
-        mfma_wx32(w2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
-        __syncthreads();
-        mfma_wx32(w3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
-        __syncthreads();
-        mfma_wx32(w4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, o});
-        __syncthreads();
+        guide_forward_wave32(G, F, p, nn, act, o, lds);
         const uint32_t rows = min(64u, n - base);
         for (uint32_t k = l; k < rows * kGOut; k += 64) out[(size_t)base * kGOut + k] = o[k];
         __syncthreads();

@@ -271,6 +271,42 @@ __device__ __forceinline__ void mfma_wx32(const Frags32<KT32, TAIL, OUT_TILES>& 
     }
 }
 
+// mfma_wx32 with the A fragments read from W (global, L2-resident) inside
+// the loop instead of held in registers: the same MFMA sequence, so the
+// same sums, for kernels that cannot spare 128 VGPRs for the weights.
+template <int KT32, bool TAIL, int OUT_TILES, class Epi>
+__device__ __forceinline__ void mfma_wx32_mem(const uint16_t* __restrict__ W, const uint16_t* in, int in_stride,
+                                              Epi epi) {
+    constexpr int kCols = 32 * KT32 + (TAIL ? 16 : 0);
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, grp = l >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const uint16_t* x = in + (16 * nt + row) * in_stride;
+        g_bf8 b[KT32];
+#pragma unroll
+        for (int kt = 0; kt < KT32; ++kt) b[kt] = *reinterpret_cast<const g_bf8*>(x + 32 * kt + 8 * grp);
+        g_s4 bt = {0, 0, 0, 0};
+        if (TAIL) bt = *reinterpret_cast<const g_s4*>(x + 32 * KT32 + 4 * grp);
+#pragma unroll
+        for (int mt = 0; mt < OUT_TILES; ++mt) {
+            const uint16_t* w = W + (16 * mt + row) * kCols;
+            g_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < KT32; ++kt)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const g_bf8*>(w + 32 * kt + 8 * grp),
+                                                              b[kt], acc, 0, 0, 0);
+            if (TAIL) {
+                g_f4 t = {0.f, 0.f, 0.f, 0.f};
+                t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const g_s4*>(w + 32 * KT32 + 4 * grp),
+                                                              bt, t, 0, 0, 0);
+                acc += t;
+            }
+            epi(16 * mt + 4 * grp, 16 * nt + row, acc);
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
 }
@@ -315,11 +351,47 @@ __device__ __forceinline__ void guide_output(const uint16_t* W, const float* bia
     mfma_wx<4, 3>(W, in, in_stride, OutputEpi{bias, out});
 }
 
+// The A fragments of all four layers (K = 80 as 2 x 32 + 16, then 2 x 32):
+// 128 VGPRs, loaded once per wave and reused for every 64 samples.
+struct GuideFrags {
+    Frags32<2, true, 4> w1;
+    Frags32<2, false, 4> w2, w3;
+    Frags32<2, false, 3> w4;
+};
+
+__device__ __forceinline__ void load_guide_frags(const GuideView& G, GuideFrags& F) {
+    load_frags32(G.wbf + kGW1, F.w1);
+    load_frags32(G.wbf + kGW2, F.w2);
+    load_frags32(G.wbf + kGW3, F.w3);
+    load_frags32(G.wbf + kGW4, F.w4);
+}
+
 // Runs the MLP for the 64 lanes' samples of one wave (the block must be that
 // one wave: the layers are separated by block barriers).  Leaves the 40
 // outputs of lane l's sample at out[l * 40 ..] (out may alias the first
 // 64 * kGStrideIn uint16 of lds).  `lds` holds 64 * (kGStrideIn +
-// kGStrideHid) uint16 values.
+// kGStrideHid) uint16 values.  Every forward evaluation of the guide (render,
+// pdf, sampling, tn_guide_forward) runs this arithmetic, so a density the
+// renderer sampled with is the density tn_guide_pdf reports.
+__device__ __forceinline__ void guide_forward_wave32(const GuideView& G, const GuideFrags& F, float3 p,
+                                                     float3 n, bool active, float* out, uint16_t* lds) {
+    const int l = threadIdx.x & 63;
+    uint16_t* a = lds;                    // [64][kGStrideIn]
+    uint16_t* b = lds + 64 * kGStrideIn;  // [64][kGStrideHid]
+    guide_features(G, p, n, active, a + l * kGStrideIn);
+    __syncthreads();
+    mfma_wx32(F.w1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
+    __syncthreads();
+    mfma_wx32(F.w2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
+    __syncthreads();
+    mfma_wx32(F.w3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
+    __syncthreads();
+    mfma_wx32(F.w4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, out});
+    __syncthreads();
+}
+
+// As guide_forward_wave32 with the weights read from memory layer by layer
+// (same sums, far fewer registers: for the shade and evaluation kernels).
 __device__ __forceinline__ void guide_forward_wave(const GuideView& G, float3 p, float3 n, bool active,
                                                    float* out, uint16_t* lds) {
     const int l = threadIdx.x & 63;
@@ -327,13 +399,13 @@ __device__ __forceinline__ void guide_forward_wave(const GuideView& G, float3 p,
     uint16_t* b = lds + 64 * kGStrideIn;  // [64][kGStrideHid]
     guide_features(G, p, n, active, a + l * kGStrideIn);
     __syncthreads();
-    guide_hidden<5>(G.wbf + kGW1, G.theta + kGOffB1, a, kGStrideIn, b, kGStrideHid);
+    mfma_wx32_mem<2, true, 4>(G.wbf + kGW1, a, kGStrideIn, HiddenEpi{G.theta + kGOffB1, b, kGStrideHid});
     __syncthreads();
-    guide_hidden<4>(G.wbf + kGW2, G.theta + kGOffB2, b, kGStrideHid, a, kGStrideIn);
+    mfma_wx32_mem<2, false, 4>(G.wbf + kGW2, b, kGStrideHid, HiddenEpi{G.theta + kGOffB2, a, kGStrideIn});
     __syncthreads();
-    guide_hidden<4>(G.wbf + kGW3, G.theta + kGOffB3, a, kGStrideIn, b, kGStrideHid);
+    mfma_wx32_mem<2, false, 4>(G.wbf + kGW3, a, kGStrideIn, HiddenEpi{G.theta + kGOffB3, b, kGStrideHid});
     __syncthreads();
-    guide_output(G.wbf + kGW4, G.theta + kGOffB4, b, kGStrideHid, out);
+    mfma_wx32_mem<2, false, 3>(G.wbf + kGW4, b, kGStrideHid, OutputEpi{G.theta + kGOffB4, out});
     __syncthreads();
 }
 
