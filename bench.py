@@ -345,11 +345,31 @@ def tree_probe():
             e1.record()
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1)
+            # the first 2^18 rays alone: a launch that small is bound by its slowest rays
+            ns = min(n, 1 << 18)
+            ts, ds_ = to[:ns].contiguous(), td[:ns].contiguous()
+            tms = tm[:ns].contiguous() if tm is not None else None
+
+            def go_small():
+                if tm is None:
+                    T.check(L.tn_intersect(tree.handle, T.ptr(ts), T.ptr(ds_), ns, T.ptr(ids), T.ptr(r),
+                                           T.ptr(s_), T.ptr(t_), 0, st))
+                else:
+                    T.check(L.tn_occluded(tree.handle, T.ptr(ts), T.ptr(ds_), T.ptr(tms), ns, T.ptr(occ), 0, st))
+            go_small()
+            e0.record()
+            go_small()
+            e1.record()
+            torch.cuda.synchronize()
+            ms_small = e0.elapsed_time(e1)
+            go(0)  # restore the full batch's outputs
+            torch.cuda.synchronize()
             c = cnt.astype(np.float64) / n
-            res = {"n": n, "ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1)}
+            res = {"n": n, "ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1),
+                   "first_2p18_ms": round(ms_small, 3), "max_leaves_one_ray": int(cnt[7]),
+                   "leaves_per_ray": round(c[1], 2)}
             if tm is None:
-                res.update({"inner_per_ray": round(c[0], 2), "leaves_per_ray": round(c[1], 2),
-                            "tests_per_ray": round(c[3], 2)})
+                res.update({"inner_per_ray": round(c[0], 2), "tests_per_ray": round(c[3], 2)})
             return res, ids.cpu().numpy().view(np.uint32), r.cpu().numpy()
 
         res_cam, ids, r = run(co, cd)

@@ -156,7 +156,8 @@ int tn_occluded(tn_scene* scene, const float* o3, const float* d3, const float* 
 /* Work counters of the launches made with TN_ISECT_COUNT since the last reset:
  * [0] inner nodes visited, [1] leaves visited, [2] triangle references read,
  * [3] ray-triangle tests, [4] rays, [5] stack restarts, [6] tests that read
- * the whole 64-byte triangle record (the rest stop after 32 bytes), [7] 0.
+ * the whole 64-byte triangle record (the rest stop after 32 bytes), [7] the
+ * most leaves any single ray visited.
  * reset != 0 zeroes them.  Synchronises the device. */
 int tn_get_counters(tn_scene* scene, uint64_t* out8, int reset);
 

@@ -49,8 +49,8 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
                                                  uint8_t* __restrict__ occ,
                                                  unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
-    __shared__ uint32_t red[7];
-    if (COUNT && threadIdx.x < 7) red[threadIdx.x] = 0;
+    __shared__ uint32_t red[8];
+    if (COUNT && threadIdx.x < 8) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
     uint2* my = stk + threadIdx.x;
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -120,6 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
         atomicAdd(&red[6], cnt.full);
         __syncthreads();
         if (threadIdx.x < 7) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+        if (threadIdx.x == 7) atomicMax(&counters[7], (unsigned long long)red[7]);
     }
 }
 
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
             const bool hit = traversed && T.best_id != kInvalidId;
+            if (COUNT) atomicMax(&red[7], T.visits);  // leaves of the longest traversal
             if (ANY) {
                 occ[k] = hit ? 1 : 0;
             } else {
