@@ -366,10 +366,10 @@ def tree_probe():
             torch.cuda.synchronize()
             c = cnt.astype(np.float64) / n
             res = {"n": n, "ms": round(ms, 3), "mrays_s": round(n / ms / 1e3, 1),
-                   "first_2p18_ms": round(ms_small, 3), "max_leaves_one_ray": int(cnt[7]),
-                   "leaves_per_ray": round(c[1], 2)}
-            if tm is None:
-                res.update({"inner_per_ray": round(c[0], 2), "tests_per_ray": round(c[3], 2)})
+                   "first_2p18_ms": round(ms_small, 3)}
+            if tm is None:  # tn_occluded keeps no counters
+                res.update({"inner_per_ray": round(c[0], 2), "leaves_per_ray": round(c[1], 2),
+                            "tests_per_ray": round(c[3], 2), "max_leaves_one_ray": int(cnt[7])})
             return res, ids.cpu().numpy().view(np.uint32), r.cpu().numpy()
 
         res_cam, ids, r = run(co, cd)

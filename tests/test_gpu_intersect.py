@@ -206,6 +206,8 @@ def test_torch_device_batch_and_counters(gpu):
     cnt = np.zeros(8, np.uint64)
     T.check(T.lib().tn_get_counters(tree.handle, cnt.ctypes.data, 1))
     assert cnt[4] == len(o)
+    # the longest traversal lies between the mean and the total leaf count
+    assert cnt[1] <= cnt[7] * cnt[4] and cnt[7] <= cnt[1]
     oc = O.intersect_count(tree.triangle_records(), tree.flat_nodes(), tree.box(), o, d)
     # the oracle counts the same near-first early-exit traversal (with an
     # unbounded stack; restarts of the short stack only add revisits)

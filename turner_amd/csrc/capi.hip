@@ -104,7 +104,10 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
                     out_s[ray] = hit ? T.best_s : 0.f;
                     out_t[ray] = hit ? T.best_t : 0.f;
                 }
-                if (COUNT) cnt.restarts += T.restarts;
+                if (COUNT) {
+                    cnt.restarts += T.restarts;
+                    atomicMax(&red[7], T.visits);  // leaves of the longest traversal
+                }
                 ray = kNone;
             }
         }
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
     __shared__ unsigned long long slots[kBlock];
-    __shared__ uint32_t red[7];
-    if (COUNT && threadIdx.x < 7) red[threadIdx.x] = 0;
+    __shared__ uint32_t red[8];  // [0..6] sums, [7] longest traversal in leaves
+    if (COUNT && threadIdx.x < 8) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
     const uint32_t per_xcd = gridDim.x / kXcds;
     const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -185,6 +188,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         atomicAdd(&red[6], cnt.full);
         __syncthreads();
         if (threadIdx.x < 7) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+        if (threadIdx.x == 7) atomicMax(&counters[7], (unsigned long long)red[7]);
     }
 }
 
