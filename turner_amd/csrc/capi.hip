@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     float* __restrict__ out_r, float* __restrict__ out_s, float* __restrict__ out_t,
     uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status) {
     __shared__ uint2 stk[kStack * kBlock];
-    __shared__ unsigned long long slots[kBlock];
+    __shared__ CoopLds wl[kBlock / 64];
     __shared__ uint32_t red[8];  // [0..6] sums, [7] longest traversal in leaves
     if (COUNT && threadIdx.x < 8) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     uint32_t nrays = 0;
     TraversalCounters cnt;
     coop_loop<kStack, EXH, MT, ANY, COUNT>(
-        sc, stk + threadIdx.x, kBlock, slots + (threadIdx.x & ~63u),
+        sc, stk + threadIdx.x, kBlock, wl + threadIdx.x / 64,
         [&](uint32_t& b, uint32_t& e) {
             if (given || first >= n) return false;
             given = true;

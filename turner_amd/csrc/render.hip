@@ -349,14 +349,14 @@ template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
     RenderArgs A, Queues Q, int level, int buf) {
     __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ unsigned long long slots[kRBlock];
+    __shared__ CoopLds wl[kRBlock / 64];
     const uint32_t n = min(Q.cnt[level], Q.cap);  // overflow is reported, never read
     const float4* qo = Q.o[buf];
     const float4* qd = Q.d[buf];
     uint4* hit = Q.hit;
     TraversalCounters cnt;
     coop_loop<kRStack, EXH, false, false, false>(
-        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        A.sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
         DynamicChunks{n, Q.xwork + 16 * kXcds * level},
         [&](uint32_t k, float3& o, float3& d, float&, uint32_t& scene) {
             const float4 a = qo[k], b = qd[k];
@@ -377,12 +377,12 @@ template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
     RenderArgs A, Queues Q, int level) {
     __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ unsigned long long slots[kRBlock];
+    __shared__ CoopLds wl[kRBlock / 64];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     const int sb = level & 1;
     TraversalCounters cnt;
     coop_loop<kRStack, EXH, false, true, false>(
-        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        A.sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
         DynamicChunks{n, Q.xwork + 16 * kXcds * (kMaxLevels + level)},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
             const float4 a = Q.so[sb][k], b = Q.sd[sb][k];
@@ -580,13 +580,13 @@ template <bool EXH>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_rt_shadow(
     RenderArgs A, Queues Q, int level, int buf) {
     __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ unsigned long long slots[kRBlock];
+    __shared__ CoopLds wl[kRBlock / 64];
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     const int ob = buf ^ 1;
     const int sb = level & 1;
     TraversalCounters cnt;
     coop_loop<kRStack, EXH, false, true, false>(
-        A.sv, stk + threadIdx.x, kRBlock, slots + (threadIdx.x & ~63u),
+        A.sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
         DynamicChunks{n, Q.xwork + 16 * kXcds * (kMaxLevels + level)},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
             const float4 a = Q.so[sb][k], b = Q.sd[sb][k];

@@ -526,17 +526,48 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Inclusive scan (sum, or maximum with MAX) over the 64 lanes of a wave, all
+// lanes active: row shifts by 1, 2, 4, 8 within each 16-lane row (lanes whose
+// source falls outside the row add the identity 0), then lane 15 of each row
+// into rows 1 and 3 and lane 31 into rows 2 and 3.  Six DPP instructions.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
+#define TN_SCAN_STEP(ctrl, rows, bound)                                                          \
+    {                                                                                            \
+        const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, ctrl, rows, 0xf, bound); \
+        x = MAX ? max(x, t) : x + t;                                                             \
+    }
+    TN_SCAN_STEP(0x111, 0xf, true)   // row_shr:1
+    TN_SCAN_STEP(0x112, 0xf, true)   // row_shr:2
+    TN_SCAN_STEP(0x114, 0xf, true)   // row_shr:4
+    TN_SCAN_STEP(0x118, 0xf, true)   // row_shr:8
+    TN_SCAN_STEP(0x142, 0xa, false)  // row_bcast:15
+    TN_SCAN_STEP(0x143, 0xc, false)  // row_bcast:31
+#undef TN_SCAN_STEP
+    return x;
+}
+
+// LDS of one wave for the cooperative loop: the closest-hit key of each lane's
+// leaf and the start markers of the pooled pairs of one round.
+struct CoopLds {
+    unsigned long long slot[64];
+    uint32_t mark[64];
+};
+
 
 // Wave-cooperative traversal loop.  Every round, each lane with a ray
 // descends to its next leaf; then the wave pools the (ray, triangle
-// reference) pairs of all its lanes' leaves and tests them 64 at a time --
-// lane j takes pair base + j, finds the owning lane by binary search over the
-// exclusive prefix of leaf sizes and fetches the owner's ray with
-// ds_bpermute -- so a round costs ceil(sum of leaf sizes / 64) tests instead
-// of the largest leaf.  A hit is folded into the owner's 64-bit LDS slot with
-// atomicMin on (r, position in leaf): the minimum is the first of the leaf's
-// closest hits in leaf order, i.e. the reference's strict-< scan
-// (kdtree.cpp:587-605); the owner re-tests that triangle to recover r, s, t.
+// reference) pairs of all its lanes' leaves and tests them 64 at a time, so a
+// round costs ceil(sum of leaf sizes / 64) tests instead of the largest leaf.
+// Pair p belongs to the lane whose range [excl, incl) of the prefix sum of
+// leaf sizes holds it.  In the round of pairs base..base+63 every lane whose
+// range meets the round writes a marker (round stamp, lane) at its first pair
+// of the round; a prefix maximum of the markers over the lanes gives each
+// pair its owner, whose ray is fetched with ds_bpermute.  A hit is folded
+// into the owner's 64-bit LDS slot with atomicMin on (r, pair index): the
+// minimum is the first of the leaf's closest hits in leaf order, i.e. the
+// reference's strict-< scan (kdtree.cpp:587-605); the owner re-tests that
+// triangle to recover r, s, t.
 // Finished lanes take the next rays of the wave's chunk.
 //   chunk(begin, end): called by all lanes of the wave; returns false when no
 //                      work is left (may update begin/end);
@@ -544,7 +575,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 //                      the ray's tree in a forest, 0 otherwise).
 template <int S, bool EXH, bool MT, bool ANY, bool COUNT, class Chunk, class Load, class Store>
 __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int stride,
-                                          unsigned long long* wslot, Chunk chunk, Load load,
+                                          CoopLds* wl, Chunk chunk, Load load,
                                           Store store, TraversalCounters& cnt) {
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -578,35 +609,30 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
             lcnt = nd.y >> 2;
         }
         // phase 2: pooled leaf tests
-        uint32_t incl = lcnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o, 64);
-            if ((int)lane >= o) incl += v;
-        }
+        const uint32_t incl = wave_scan<false>(lcnt);
         const uint32_t excl = incl - lcnt;
-        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        const uint32_t rbase = loff - excl;  // refs index of pair p of this lane: rbase + p
         const float bound_own = ANY ? tmx : T.best_r;
-        wslot[lane] = ~0ull;
+        wl->slot[lane] = ~0ull;
+        wl->mark[lane] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        uint32_t stamp = 0;
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t item = base + lane;
-            uint32_t own = 0;
-#pragma unroll
-            for (uint32_t step = 32; step >= 1; step >>= 1) {
-                const uint32_t cand = own + step;
-                const uint32_t pc = __shfl(excl, (int)(cand & 63u), 64);
-                if (cand < 64u && pc <= item) own = cand;
-            }
-            const uint32_t loc = item - __shfl(excl, (int)own, 64);
-            const uint32_t off = __shfl(loff, (int)own, 64);
-            const float3 ro = make_float3(__shfl(T.o.x, (int)own, 64), __shfl(T.o.y, (int)own, 64),
-                                          __shfl(T.o.z, (int)own, 64));
-            const float3 rd = make_float3(__shfl(T.d.x, (int)own, 64), __shfl(T.d.y, (int)own, 64),
-                                          __shfl(T.d.z, (int)own, 64));
-            const float bnd = __shfl(bound_own, (int)own, 64);
+            stamp += 64u;  // above every marker of the earlier rounds
+            if (lcnt != 0 && excl < base + 64u && incl > base)
+                wl->mark[excl > base ? excl - base : 0u] = stamp | lane;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const int own = (int)(wave_scan<true>(wl->mark[lane]) & 63u);
+            const uint32_t rb = __shfl(rbase, own, 64);
+            const float3 ro = make_float3(__shfl(T.o.x, own, 64), __shfl(T.o.y, own, 64),
+                                          __shfl(T.o.z, own, 64));
+            const float3 rd = make_float3(__shfl(T.d.x, own, 64), __shfl(T.d.y, own, 64),
+                                          __shfl(T.d.z, own, 64));
+            const float bnd = __shfl(bound_own, own, 64);
             if (item < total) {
-                const uint32_t id = sc.refs[off + loc];
+                const uint32_t id = sc.refs[rb + item];
                 float rr, ss, tt;
                 const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt)
                                     : leaf_test_ref<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt);
@@ -617,17 +643,17 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                 }
                 if (code == 2) {
                     // r >= 0 here; -0 maps to +0 so the key orders like the reference's <
-                    const uint32_t rb = __float_as_uint(rr + 0.0f);
-                    atomicMin(&wslot[own], ((unsigned long long)rb << 32) | loc);
+                    const uint32_t rbits = __float_as_uint(rr + 0.0f);
+                    atomicMin(&wl->slot[own], ((unsigned long long)rbits << 32) | item);
                 }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const unsigned long long key = wslot[lane];
+        const unsigned long long key = wl->slot[lane];
         // every active lane moves on after its leaf, empty leaves included
         if (ray != kNone) {
             if (lcnt && key != ~0ull) {
-                const uint32_t id = sc.refs[loff + (uint32_t)key];
+                const uint32_t id = sc.refs[rbase + (uint32_t)key];
                 float rr, ss, tt;
                 if (MT) leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
                 else leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
