@@ -115,8 +115,9 @@ def main():
     T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
     inner, leaves, refs, tests, rays, restarts, full = (int(x) for x in cnt[:7])
     # bytes: ray in (24) + hit out (16) + 8 per node visited (inner + leaf)
-    # + 4 per triangle reference + 32 per test + 32 more for full-record tests
-    alg_bytes = 40 * n + 8 * (inner + leaves) + 4 * refs + 32 * tests + 32 * full
+    # + 32 per test + 32 more for full-record tests (the records are stored
+    # per triangle reference, so a test reads no separate reference word)
+    alg_bytes = 40 * n + 8 * (inner + leaves) + 32 * tests + 32 * full
 
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)

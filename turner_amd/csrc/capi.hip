@@ -213,6 +213,20 @@ __global__ void k_tonemap(const float4* __restrict__ acc, float4* __restrict__ o
     out[i] = c;
 }
 
+// One thread per triangle reference: copies its record into the two
+// reference-ordered half arrays.
+__global__ __launch_bounds__(256) void k_gather_leaf_records(const float4* __restrict__ tris,
+                                                             const uint32_t* __restrict__ refs,
+                                                             size_t nrefs, float4* __restrict__ out) {
+    const size_t r = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= nrefs) return;
+    const float4* src = tris + 4 * (size_t)refs[r];
+    out[2 * r] = src[0];
+    out[2 * r + 1] = src[1];
+    out[2 * nrefs + 2 * r] = src[2];
+    out[2 * nrefs + 2 * r + 1] = src[3];
+}
+
 int check_device(int* dev) {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -235,6 +249,7 @@ void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_nodes);
     (void)hipFree(sc->d_refs);
     (void)hipFree(sc->d_tris);
+    (void)hipFree(sc->d_lrec);
     (void)hipFree(sc->d_normals);
     (void)hipFree(sc->d_albedo);
     (void)hipFree(sc->d_emission);
@@ -270,6 +285,9 @@ int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const
         rec[4 * (size_t)i + 3] = make_float4(p[9], p[10], p[11], p[15]);
     }
     if ((rc = dev_upload(&sc->d_tris, rec.data(), rec.size()))) return rc;
+    sc->nrefs = (uint32_t)sc->tree.refs.size();
+    if ((rc = dev_upload<float4>(&sc->d_lrec, nullptr, 4 * (size_t)sc->nrefs))) return rc;
+    if ((rc = gather_leaf_records(sc->d_tris, sc->d_refs, sc->nrefs, sc->d_lrec))) return rc;
     std::vector<float> nrm((size_t)n * 9);
     if (normals9) {
         std::memcpy(nrm.data(), normals9, nrm.size() * sizeof(float));
@@ -327,6 +345,16 @@ int prepare_triangles(tn_scene* sc, const float* verts9, uint32_t n) {
 }
 
 }  // namespace
+
+int gather_leaf_records(const float4* tris, const uint32_t* refs, size_t nrefs, float4* out) {
+    if (nrefs == 0) return TN_OK;
+    const size_t blocks = (nrefs + 255) / 256;
+    if (blocks > 0x7FFFFFFFu) return set_error(TN_ERR_INVALID, "too many triangle references: %zu", nrefs);
+    hipLaunchKernelGGL(k_gather_leaf_records, dim3((uint32_t)blocks), dim3(256), 0, 0, tris, refs, nrefs, out);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipStreamSynchronize(0));
+    return TN_OK;
+}
 
 int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
                 uint32_t* id, float* r, float* s, float* t, uint8_t* occ, uint32_t flags,

@@ -1493,6 +1493,8 @@ struct tn_batch {
     uint2* nodes = nullptr;
     uint32_t* refs = nullptr;
     float4* tris = nullptr;
+    float4* lrec = nullptr;  // records per reference (first halves, then second halves)
+    size_t nrefs = 0;
     float* normals = nullptr;
     float4* albedo = nullptr;
     float4* emission = nullptr;
@@ -1517,6 +1519,8 @@ struct tn_batch {
         g.sv.nodes = nodes;
         g.sv.refs = refs;
         g.sv.tris = tris;
+        g.sv.lra = lrec;
+        g.sv.lrb = lrec + 2 * nrefs;
         for (int k = 0; k < 3; ++k) g.sv.bmin[k] = g.sv.bmax[k] = 0.f;
         g.sv.roots = roots;
         g.normals = normals;
@@ -1536,7 +1540,8 @@ namespace tn {
 namespace {
 
 void free_batch(tn_batch* b) {
-    for (void* p : {(void*)b->nodes, (void*)b->refs, (void*)b->tris, (void*)b->normals, (void*)b->albedo,
+    for (void* p : {(void*)b->nodes, (void*)b->refs, (void*)b->tris, (void*)b->lrec, (void*)b->normals,
+                    (void*)b->albedo,
                     (void*)b->emission, (void*)b->emit_ids, (void*)b->emit_cdf, (void*)b->roots,
                     (void*)b->status})
         if (p) (void)hipFree(p);
@@ -1795,6 +1800,13 @@ int tn_batch_create(tn_scene* const* scenes, uint32_t n, tn_batch** out) {
             (e = hipMemcpy(b->albedo + t0, sc->d_albedo, k * 16, hipMemcpyDeviceToDevice)) != hipSuccess ||
             (e = hipMemcpy(b->emission + t0, sc->d_emission, k * 16, hipMemcpyDeviceToDevice)) != hipSuccess)
             return fail(e, "hipMemcpy");
+    }
+    b->nrefs = refs.size();
+    if ((e = hipMalloc(&b->lrec, std::max<size_t>(b->nrefs, 1) * 64)) != hipSuccess) return fail(e, "hipMalloc");
+    if (int rc = gather_leaf_records(b->tris, b->refs, b->nrefs, b->lrec)) {
+        free_batch(b);
+        delete b;
+        return rc;
     }
     *out = b;
     return TN_OK;

@@ -25,6 +25,8 @@ struct tn_scene {
     uint2* d_nodes = nullptr;
     uint32_t* d_refs = nullptr;
     float4* d_tris = nullptr;
+    float4* d_lrec = nullptr;    // records per reference: 2 x nrefs first halves, then second halves
+    uint32_t nrefs = 0;
     float* d_normals = nullptr;  // 9 per triangle
     float4* d_albedo = nullptr;
     float4* d_emission = nullptr;
@@ -40,6 +42,8 @@ struct tn_scene {
         v.nodes = d_nodes;
         v.refs = d_refs;
         v.tris = d_tris;
+        v.lra = d_lrec;
+        v.lrb = d_lrec ? d_lrec + 2 * (size_t)nrefs : nullptr;
         for (int k = 0; k < 3; ++k) {
             v.bmin[k] = tree.box[k];
             v.bmax[k] = tree.box[3 + k];
@@ -58,6 +62,11 @@ int hip_error(hipError_t e, const char* what);
         hipError_t e_ = (call);                                   \
         if (e_ != hipSuccess) return ::tn::hip_error(e_, #call);  \
     } while (0)
+
+// Writes out[2r], out[2r+1] = tris[4 refs[r]], tris[4 refs[r] + 1] and
+// out[2 nrefs + 2r], out[2 nrefs + 2r + 1] = tris[4 refs[r] + 2], tris[4 refs[r] + 3]
+// for r < nrefs (every refs[r] indexes tris), on the null stream, and waits.
+int gather_leaf_records(const float4* tris, const uint32_t* refs, size_t nrefs, float4* out);
 
 // Fixed launch shape of the traversal kernels: 256 lanes = 4 wave64.
 constexpr int kBlock = 256;

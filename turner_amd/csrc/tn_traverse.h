@@ -59,6 +59,12 @@ struct SceneView {
     const uint2* nodes;
     const uint32_t* refs;
     const float4* tris;  // 4 per triangle
+    // The same records once per triangle reference, in reference order, as
+    // two arrays of halves: lra[2r..2r+1] = (v0, uv) (n, vv) and
+    // lrb[2r..2r+1] = (u, uu) (v, denom) of triangle refs[r].  The pairs of a
+    // leaf are then contiguous and need no reference lookup.
+    const float4* lra;
+    const float4* lrb;
     float bmin[3];
     float bmax[3];
     const SceneRoot* roots = nullptr;  // forest: per-scene roots (bmin/bmax unused)
@@ -98,12 +104,13 @@ __device__ __forceinline__ bool certainly_negative(float num, float den) {
 // A candidate is also rejected when r is not below `bound` (closest hit: the
 // current best, which it must beat strictly) or, with ANY, above it.
 // Returns 2 on a hit, 1 for a miss decided after reading the whole record,
-// 0 for a miss decided from its first 32 bytes.
+// 0 for a miss decided from its first 32 bytes.  pa: the record's first half
+// (v0, uv) (n, vv); pc: its second half (u, uu) (v, denom).
 template <bool ANY>
-__device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint32_t id, float3 o,
-                                             float3 d, float bound, float& r, float& s, float& t) {
-    const float4 a = T[4 * (size_t)id];      // v0, uv
-    const float4 b = T[4 * (size_t)id + 1];  // n, vv
+__device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ pa, const float4* __restrict__ pc,
+                                             float3 o, float3 d, float bound, float& r, float& s, float& t) {
+    const float4 a = pa[0];  // v0, uv
+    const float4 b = pa[1];  // n, vv
     const float den = b.x * d.x + b.y * d.y + b.z * d.z;
     if (den == 0.f) return 0;  // r = lowest() < 0
     const float nom = b.x * (a.x - o.x) + b.y * (a.y - o.y) + b.z * (a.z - o.z);
@@ -118,8 +125,8 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint3
     const float rr = nom / den;
     if (rr < 0) return 0;
     if (ANY ? !(rr <= bound) : !(rr < bound)) return 0;
-    const float4 c = T[4 * (size_t)id + 2];  // u, uu
-    const float4 e = T[4 * (size_t)id + 3];  // v, denom
+    const float4 c = pc[0];  // u, uu
+    const float4 e = pc[1];  // v, denom
     const float px = o.x + rr * d.x, py = o.y + rr * d.y, pz = o.z + rr * d.z;
     const float wx = px - a.x, wy = py - a.y, wz = pz - a.z;
     const float wv = wx * e.x + wy * e.y + wz * e.z;
@@ -150,11 +157,11 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ T, uint3
 // Moeller & Trumbore 1997 in FP32 on the record's v0, e1 = u, e2 = v.
 // Returns 2 on a hit, 1 on a miss (the test always reads v0, u and v).
 template <bool ANY>
-__device__ __forceinline__ int leaf_test_mt(const float4* __restrict__ T, uint32_t id, float3 o,
-                                            float3 d, float bound, float& r, float& s, float& t) {
-    const float4 a = T[4 * (size_t)id];
-    const float4 c = T[4 * (size_t)id + 2];
-    const float4 e = T[4 * (size_t)id + 3];
+__device__ __forceinline__ int leaf_test_mt(const float4* __restrict__ pa, const float4* __restrict__ pc,
+                                            float3 o, float3 d, float bound, float& r, float& s, float& t) {
+    const float4 a = pa[0];
+    const float4 c = pc[0];
+    const float4 e = pc[1];
     const float px = d.y * e.z - d.z * e.y, py = d.z * e.x - d.x * e.z, pz = d.x * e.y - d.y * e.x;
     const float det = c.x * px + c.y * py + c.z * pz;
     if (det == 0.f) return 1;
@@ -344,8 +351,9 @@ __device__ __forceinline__ bool trav_step(Trav& T, const SceneView& sc, float tm
         const uint32_t id = sc.refs[off + k];
         float rr, ss, tt;
         const float bound = ANY ? tmax_any : T.best_r;
-        const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
-                            : leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt);
+        const float4* pa = sc.tris + 4 * (size_t)id;
+        const int code = MT ? leaf_test_mt<ANY>(pa, pa + 2, T.o, T.d, bound, rr, ss, tt)
+                            : leaf_test_ref<ANY>(pa, pa + 2, T.o, T.d, bound, rr, ss, tt);
         if (COUNT) {
             cnt.refs++;
             cnt.tests++;
@@ -453,8 +461,9 @@ __device__ __forceinline__ bool trav_tick(Trav& T, const SceneView& sc, float tm
         --T.leaf_left;
         float rr, ss, tt;
         const float bound = ANY ? tmax_any : T.best_r;
-        const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt)
-                            : leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound, rr, ss, tt);
+        const float4* pa = sc.tris + 4 * (size_t)id;
+        const int code = MT ? leaf_test_mt<ANY>(pa, pa + 2, T.o, T.d, bound, rr, ss, tt)
+                            : leaf_test_ref<ANY>(pa, pa + 2, T.o, T.d, bound, rr, ss, tt);
         if (COUNT) {
             cnt.refs++;
             cnt.tests++;
@@ -617,14 +626,18 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
         wl->slot[lane] = ~0ull;
         wl->mark[lane] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // owner of pair base + lane; called with all lanes active
         uint32_t stamp = 0;
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t item = base + lane;
+        auto owner_of = [&](uint32_t base) {
             stamp += 64u;  // above every marker of the earlier rounds
             if (lcnt != 0 && excl < base + 64u && incl > base)
                 wl->mark[excl > base ? excl - base : 0u] = stamp | lane;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const int own = (int)(wave_scan<true>(wl->mark[lane]) & 63u);
+            return (int)(wave_scan<true>(wl->mark[lane]) & 63u);
+        };
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t item = base + lane;
+            const int own = owner_of(base);
             const uint32_t rb = __shfl(rbase, own, 64);
             const float3 ro = make_float3(__shfl(T.o.x, own, 64), __shfl(T.o.y, own, 64),
                                           __shfl(T.o.z, own, 64));
@@ -632,10 +645,10 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                                           __shfl(T.d.z, own, 64));
             const float bnd = __shfl(bound_own, own, 64);
             if (item < total) {
-                const uint32_t id = sc.refs[rb + item];
+                const size_t r = rb + item;  // reference slot of the pair
                 float rr, ss, tt;
-                const int code = MT ? leaf_test_mt<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt)
-                                    : leaf_test_ref<ANY>(sc.tris, id, ro, rd, bnd, rr, ss, tt);
+                const int code = MT ? leaf_test_mt<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt)
+                                    : leaf_test_ref<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt);
                 if (COUNT) {
                     cnt.refs++;
                     cnt.tests++;
@@ -653,10 +666,11 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
         // every active lane moves on after its leaf, empty leaves included
         if (ray != kNone) {
             if (lcnt && key != ~0ull) {
-                const uint32_t id = sc.refs[rbase + (uint32_t)key];
+                const size_t r = rbase + (uint32_t)key;
+                const uint32_t id = sc.refs[r];
                 float rr, ss, tt;
-                if (MT) leaf_test_mt<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
-                else leaf_test_ref<ANY>(sc.tris, id, T.o, T.d, bound_own, rr, ss, tt);
+                if (MT) leaf_test_mt<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, T.o, T.d, bound_own, rr, ss, tt);
+                else leaf_test_ref<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, T.o, T.d, bound_own, rr, ss, tt);
                 T.best_r = rr;
                 T.best_s = ss;
                 T.best_t = tt;
