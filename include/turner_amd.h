@@ -70,7 +70,10 @@ typedef struct {
  * precomputes the triangles (lib/triangle.h:22-39), builds the SAH kd-tree on
  * the host with the reference's algorithm and constants (lib/kdtree.cpp:124-467)
  * and uploads nodes, triangle references, triangle records and shading data
- * to the device.  verts9: 9 floats per triangle (v0 v1 v2).  normals9: vertex
+ * to the device.  Device memory: 8 B per node, 4 B per triangle reference,
+ * 64 B per triangle record plus a 64-byte copy of the record per triangle
+ * reference (reference order, read by the traversal), 68 B of shading data per
+ * triangle.  verts9: 9 floats per triangle (v0 v1 v2).  normals9: vertex
  * normals (NULL: face normals).  albedo4: RGBA diffuse per triangle (NULL:
  * 0.8 grey, alpha 1).  emission4: RGBA emission per triangle (NULL: none).
  */
