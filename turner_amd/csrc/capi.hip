@@ -127,8 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
     }
 }
 
-// Wave-cooperative variant of k_rays (coop_loop): same chunking and XCD
-// mapping, pooled leaf tests.  WPE caps the VGPRs so that WPE waves fit on
+// Wave-cooperative variant of k_rays (coop_loop): the same XCD mapping, one
+// range of `chunk` rays per wave, pooled leaf tests.  WPE caps the VGPRs so that WPE waves fit on
 // each SIMD.
 constexpr int kCoopWavesPerEU = 7;  // measured best (5: 964, 6: 964, 7: 1036, 8: 942 Mrays/s)
 template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU>
@@ -362,15 +362,20 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     if (n == 0) return TN_OK;
     if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
     const SceneView v = sc->view();
-    // rays per wave: up to kChunk, fewer when the batch would not give every
-    // resident wave (7 per SIMD) work
-    const uint32_t chunk = rays_per_wave(n, sc->cus);
-    const uint32_t waves = (n + chunk - 1) / chunk;
-    const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
                cnt = flags & TN_ISECT_COUNT, any = occ != nullptr,
                tick = flags & TN_ISECT_TICK, leaf = flags & TN_ISECT_LEAF;
+    // Rays per wave.  The cooperative kernel refills its lanes from one range
+    // per wave, so every resident wave (7 per SIMD) gets an equal share of the
+    // batch and all finish together (fixed 512-ray chunks left the last of
+    // several generations of waves running on part of the chip); at least 64.
+    // The per-lane kernels take up to kChunk rays per wave.
+    const uint32_t slots = (sc->cus ? sc->cus : 256u) * 4u * kResidentWavesPerSimd;
+    const uint32_t chunk = (tick || leaf) ? rays_per_wave(n, sc->cus)
+                                          : std::max(64u, (uint32_t)(((uint64_t)n + slots - 1) / slots));
+    const uint32_t waves = (uint32_t)(((uint64_t)n + chunk - 1) / chunk);
+    const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     auto go = [&](auto kern) {
         kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, chunk, id, r, s, t, occ, sc->d_counters,
                                          sc->d_status);
