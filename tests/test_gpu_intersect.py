@@ -244,6 +244,62 @@ def test_full_config_ray_batch(gpu):
     assert_bit_exact(got, ref, "full subset")
 
 
+def test_work_slots_cover_every_ray_once(gpu):
+    """Batches of more than 256 rays per resident wave take their chunks from
+    the scene's work slots: the rays counter equals n (every ray traversed
+    exactly once), back-to-back launches agree, 20 launches alternating over
+    two streams (more than the 16 slots, so slots are reused behind their
+    events) agree, and a subset is bit-exact against the oracle."""
+    import torch
+    sc = scenes.intersect_scene(level=5)
+    tree = T.KDTree(sc["verts"])
+    _, cus = T.device_info()
+    n = max(1 << 21, cus * 28 * 256 + 4321)  # above the threshold, ragged last chunk
+    print(f"[slots] n={n}", flush=True)
+    o, d = scenes.random_rays(n, seed=11)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    isect = T.KDTreeIntersection(tree)
+    T.check(T.lib().tn_get_counters(tree.handle, np.zeros(8, np.uint64).ctypes.data, 1))
+    isect.intersect_batch(to, td, TN_ISECT_COUNT)
+    torch.cuda.synchronize()
+    cnt = np.zeros(8, np.uint64)
+    T.check(T.lib().tn_get_counters(tree.handle, cnt.ctypes.data, 1))
+    assert cnt[4] == n
+    a = isect.intersect_batch(to, td)
+    b = isect.intersect_batch(to, td)
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for k in range(20):
+        with torch.cuda.stream(streams[k % 2]):
+            outs.append(isect.intersect_batch(to, td))
+    torch.cuda.synchronize()
+    for res in outs:
+        for x, y in zip(res, a):
+            assert torch.equal(x, y)
+    ids = a[0].cpu().numpy().view(np.uint32)
+    assert np.all(ids != INV)
+    # any-hit launch of the same size: occluded iff tmax reaches the closest hit
+    tmax = a[1].clone()
+    tmax[::2] *= 0.5
+    occ = torch.empty(n, dtype=torch.uint8, device=dev)
+    T.check(T.lib().tn_occluded(tree.handle, T.ptr(to), T.ptr(td), T.ptr(tmax), n, T.ptr(occ), 0,
+                                torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    occ = occ.cpu().numpy()
+    r_even = a[1].cpu().numpy()[::2]
+    assert np.all(occ[1::2] == 1)
+    assert np.all(occ[::2][r_even > 0] == 0)
+    sub = np.random.default_rng(1).choice(n, 1 << 14, replace=False)
+    ref = O.intersect(tree.triangle_records(), tree.flat_nodes(), tree.box(), o[sub], d[sub])
+    got = tuple(x.cpu().numpy()[sub] for x in a)
+    got = (got[0].view(np.uint32), got[1], got[2], got[3])
+    assert_bit_exact(got, ref, "work slots subset")
+
+
 def test_render_tree_gives_the_same_hits(gpu):
     """TN_SCENE_EMPTY_LEAVES (rendering tree): the closest hit distance of
     every ray equals the reference tree's bit for bit; ids may differ only

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
                                                  float* __restrict__ out_s,
                                                  float* __restrict__ out_t,
                                                  uint8_t* __restrict__ occ,
-                                                 unsigned long long* counters, uint32_t* status) {
+                                                 unsigned long long* counters, uint32_t* status,
+                                                 uint32_t* /*work: cooperative kernel only*/) {
     __shared__ uint2 stk[kStack * kBlock];
     __shared__ uint32_t red[8];
     if (COUNT && threadIdx.x < 8) red[threadIdx.x] = 0;
@@ -127,16 +128,52 @@ __global__ __launch_bounds__(kBlock) void k_rays(SceneView sc, const float* __re
     }
 }
 
-// Wave-cooperative variant of k_rays (coop_loop): the same XCD mapping, one
-// range of `chunk` rays per wave, pooled leaf tests.  WPE caps the VGPRs so that WPE waves fit on
-// each SIMD.
+// Rays of one wave of the cooperative kernel.  Without `work`: one range of
+// `size` rays per wave (XCD-contiguous numbering as k_rays).  With `work` (a
+// zeroed slot: 8 per-XCD counters and a finished-wave counter, 16 words
+// apart): chunks of `size` rays, the waves of XCD x taking chunks x, x + 8,
+// ... from counter x until they run out; the last wave to finish clears the
+// slot for the next launch.  Called by all lanes of the wave.
+struct RayChunks {
+    uint32_t* work;
+    uint32_t n, size, nwaves, first, x;
+    bool given;
+    __device__ bool operator()(uint32_t& b, uint32_t& e) {
+        if (!work) {
+            if (given || first >= n) return false;
+            given = true;
+            b = first;
+            e = min(first + size, n);
+            return true;
+        }
+        uint32_t k = 0;
+        if ((threadIdx.x & 63u) == 0) k = atomicAdd(work + 16 * x, 1u);
+        const uint64_t idx = x + (uint64_t)kXcds * (uint32_t)__shfl((int)k, 0, 64);
+        if (idx * size < n) {
+            b = (uint32_t)(idx * size);
+            e = min(b + size, n);
+            return true;
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            __threadfence();  // this wave's last chunk claim precedes its finish count
+            if (atomicAdd(work + 16 * kXcds, 1u) == nwaves - 1) {
+                for (uint32_t i = 0; i < kXcds; ++i) atomicExch(work + 16 * i, 0u);
+                atomicExch(work + 16 * kXcds, 0u);
+            }
+        }
+        return false;
+    }
+};
+
+// Wave-cooperative variant of k_rays (coop_loop): rays by RayChunks, pooled
+// leaf tests.  WPE caps the VGPRs so that WPE waves fit on each SIMD.
 constexpr int kCoopWavesPerEU = 7;  // measured best (5: 964, 6: 964, 7: 1036, 8: 942 Mrays/s)
 template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rays_coop(
     SceneView sc, const float* __restrict__ o3, const float* __restrict__ d3,
     const float* __restrict__ tmax, uint32_t n, uint32_t chunk, uint32_t* __restrict__ out_id,
     float* __restrict__ out_r, float* __restrict__ out_s, float* __restrict__ out_t,
-    uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status) {
+    uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status, uint32_t* work) {
     __shared__ uint2 stk[kStack * kBlock];
     __shared__ CoopLds wl[kBlock / 64];
     __shared__ uint32_t red[8];  // [0..6] sums, [7] longest traversal in leaves
@@ -145,19 +182,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t per_xcd = gridDim.x / kXcds;
     const uint32_t lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
     const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
-    const uint32_t first = wave * chunk;
-    bool given = false;
     uint32_t nrays = 0;
     TraversalCounters cnt;
     coop_loop<kStack, EXH, MT, ANY, COUNT>(
         sc, stk + threadIdx.x, kBlock, wl + threadIdx.x / 64,
-        [&](uint32_t& b, uint32_t& e) {
-            if (given || first >= n) return false;
-            given = true;
-            b = first;
-            e = min(first + chunk, n);
-            return true;
-        },
+        RayChunks{work, n, chunk, gridDim.x * (kBlock / 64), wave * chunk, blockIdx.x % kXcds, false},
         [&](uint32_t k, float3& o, float3& d, float& tm, uint32_t&) {
             o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
             d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
@@ -259,6 +288,9 @@ void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_reflectivity);
     (void)hipFree(sc->d_counters);
     (void)hipFree(sc->d_status);
+    (void)hipFree(sc->d_work);
+    for (hipEvent_t& e : sc->work_ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const float* albedo4,
@@ -327,6 +359,9 @@ int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const
     if ((rc = dev_upload<uint32_t>(&sc->d_status, nullptr, 4))) return rc;
     TN_HIP(hipMemset(sc->d_counters, 0, 8 * sizeof(unsigned long long)));
     TN_HIP(hipMemset(sc->d_status, 0, 4 * sizeof(uint32_t)));
+    const size_t work_words = (size_t)tn_scene::kWorkSlots * tn_scene::kWorkWords;
+    if ((rc = dev_upload<uint32_t>(&sc->d_work, nullptr, work_words))) return rc;
+    TN_HIP(hipMemset(sc->d_work, 0, work_words * sizeof(uint32_t)));
     return TN_OK;
 }
 
@@ -371,14 +406,45 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     // several generations of waves running on part of the chip); at least 64.
     // The per-lane kernels take up to kChunk rays per wave.
     const uint32_t slots = (sc->cus ? sc->cus : 256u) * 4u * kResidentWavesPerSimd;
-    const uint32_t chunk = (tick || leaf) ? rays_per_wave(n, sc->cus)
-                                          : std::max(64u, (uint32_t)(((uint64_t)n + slots - 1) / slots));
-    const uint32_t waves = (uint32_t)(((uint64_t)n + chunk - 1) / chunk);
+    uint32_t chunk = (tick || leaf) ? rays_per_wave(n, sc->cus)
+                                    : std::max(64u, (uint32_t)(((uint64_t)n + slots - 1) / slots));
+    uint32_t waves = (uint32_t)(((uint64_t)n + chunk - 1) / chunk);
+    // Large cooperative launches hand out chunks of about a sixteenth of a
+    // wave's share from a work slot of the scene, so waves that draw cheap
+    // rays take more of them and all finish together.  Not under stream
+    // capture (the slot's reuse is ordered by an event recorded outside).
+    uint32_t* work = nullptr;
+    int wslot = -1;
+    std::unique_lock<std::mutex> work_lock;
+    if (!(tick || leaf) && n >= slots * 256u) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        TN_HIP(hipStreamIsCapturing(stream, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            work_lock = std::unique_lock<std::mutex>(sc->work_mu);
+            wslot = (int)(sc->work_next++ % tn_scene::kWorkSlots);
+            hipEvent_t& ev = sc->work_ev[wslot];
+            if (!ev) TN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            if (sc->work_ev_used[wslot]) TN_HIP(hipStreamWaitEvent(stream, ev, 0));
+            work = sc->d_work + (size_t)wslot * tn_scene::kWorkWords;
+            const uint64_t per = ((uint64_t)n + 16ull * slots - 1) / (16ull * slots);
+            chunk = (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(64, (per + 63) & ~63ull));
+            waves = slots;
+        }
+    }
     const uint32_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
     const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds), block(kBlock);
     auto go = [&](auto kern) {
         kern<<<grid, block, 0, stream>>>(v, o3, d3, tmax, n, chunk, id, r, s, t, occ, sc->d_counters,
-                                         sc->d_status);
+                                         sc->d_status, work);
+    };
+    // after the launch: the slot is free again once this event has passed
+    auto finish = [&]() -> int {
+        TN_HIP(hipGetLastError());
+        if (work) {
+            TN_HIP(hipEventRecord(sc->work_ev[wslot], stream));
+            sc->work_ev_used[wslot] = true;
+        }
+        return TN_OK;
     };
     const uint32_t wpe = (flags >> 24) & 15u;  // occupancy variant (tuning), 0 = default
     if (!any && !cnt && !tick && !(flags & TN_ISECT_LEAF) && !exh && !mt && wpe) {
@@ -386,8 +452,7 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
         else if (wpe == 7) go(k_rays_coop<false, false, false, false, 7>);
         else if (wpe == 8) go(k_rays_coop<false, false, false, false, 8>);
         else go(k_rays_coop<false, false, false, false, 6>);
-        TN_HIP(hipGetLastError());
-        return TN_OK;
+        return finish();
     }
 #define TN_SEL2(SC, A, C)                                                \
     if (exh) {                                                           \
@@ -423,8 +488,7 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
 #undef TN_SEL2
 #undef TN_SELC
 #undef TN_SEL
-    TN_HIP(hipGetLastError());
-    return TN_OK;
+    return finish();
 }
 
 int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,

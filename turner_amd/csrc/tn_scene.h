@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,18 @@ struct tn_scene {
     float* d_reflectivity = nullptr;   // raytracer: reflectivity factor
     unsigned long long* d_counters = nullptr;  // 8 counters
     uint32_t* d_status = nullptr;              // [0] incomplete traversals, [1] queue overflow
+    // Work counters of large tn_intersect / tn_occluded launches: a ring of
+    // slots, each 8 per-XCD chunk counters and a finished-wave counter (16
+    // words apart), zero between launches (the last wave of a launch clears
+    // its slot).  A slot is reused only after the event recorded behind its
+    // previous launch, so launches on different streams never share one.
+    static constexpr int kWorkSlots = 16;
+    static constexpr int kWorkWords = 16 * 9;
+    uint32_t* d_work = nullptr;
+    hipEvent_t work_ev[kWorkSlots] = {};
+    bool work_ev_used[kWorkSlots] = {};
+    uint32_t work_next = 0;
+    std::mutex work_mu;
 
     tn::SceneView view() const {
         tn::SceneView v;
