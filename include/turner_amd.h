@@ -143,7 +143,10 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
  * FLT_MAX on miss), s, t (barycentrics of v1, v2; 0 on miss).  Closest hit
  * with the reference's strict-< tie rule in near-first visit order.
  * stream: hipStream_t or NULL for the null stream.  Asynchronous.  At most
- * 2^31 rays per call (TN_ERR_LIMIT).
+ * 2^31 rays per call (TN_ERR_LIMIT).  Calls of more than 256 rays per
+ * resident wave use one of the scene's 16 work slots (its stream may wait for
+ * the launch that last used the slot); calls may run concurrently on several
+ * streams.
  */
 int tn_intersect(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
                  uint32_t* id, float* r, float* s, float* t, uint32_t flags, void* stream);

@@ -409,8 +409,8 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     uint32_t chunk = (tick || leaf) ? rays_per_wave(n, sc->cus)
                                     : std::max(64u, (uint32_t)(((uint64_t)n + slots - 1) / slots));
     uint32_t waves = (uint32_t)(((uint64_t)n + chunk - 1) / chunk);
-    // Large cooperative launches hand out chunks of about a sixteenth of a
-    // wave's share from a work slot of the scene, so waves that draw cheap
+    // Large cooperative launches hand out chunks of about 1/32 of a wave's
+    // share from a work slot of the scene, so waves that draw cheap
     // rays take more of them and all finish together.  Not under stream
     // capture (the slot's reuse is ordered by an event recorded outside).
     uint32_t* work = nullptr;
@@ -426,7 +426,7 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
             if (!ev) TN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
             if (sc->work_ev_used[wslot]) TN_HIP(hipStreamWaitEvent(stream, ev, 0));
             work = sc->d_work + (size_t)wslot * tn_scene::kWorkWords;
-            const uint64_t per = ((uint64_t)n + 16ull * slots - 1) / (16ull * slots);
+            const uint64_t per = ((uint64_t)n + 32ull * slots - 1) / (32ull * slots);
             chunk = (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(64, (per + 63) & ~63ull));
             waves = slots;
         }
