@@ -320,7 +320,7 @@ struct DynamicChunks {
     __device__ DynamicChunks(uint32_t n_, uint32_t* xw_) : n(n_), xw(xw_), first(true) {
         nwaves = (gridDim.x * blockDim.x) >> 6;
         wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        uint32_t c = (n + 4 * nwaves - 1) / (4 * nwaves);
+        uint32_t c = (uint32_t)(((uint64_t)n + 16ull * nwaves - 1) / (16ull * nwaves));
         c = (c + 63u) & ~63u;
         size = c < 64u ? 64u : (c > (uint32_t)kChunk ? (uint32_t)kChunk : c);
         nchunks = (n + size - 1) / size;
