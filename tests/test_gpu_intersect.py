@@ -300,6 +300,37 @@ def test_work_slots_cover_every_ray_once(gpu):
     assert_bit_exact(got, ref, "work slots subset")
 
 
+def test_large_batch_in_captured_graph(gpu):
+    """Under stream capture a batch above the work-slot threshold takes the
+    static split: the captured launch, replayed three times with eager work
+    in between, gives the eager (work-slot) result every time."""
+    import torch
+    sc = scenes.intersect_scene(level=4)
+    tree = T.KDTree(sc["verts"])
+    _, cus = T.device_info()
+    n = cus * 28 * 256 + 999
+    o, d = scenes.random_rays(n, seed=21)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    ref = T.KDTreeIntersection(tree).intersect_batch(to, td)
+    ids = torch.empty(n, dtype=torch.int32, device=dev)
+    r, s, t = (torch.empty(n, dtype=torch.float32, device=dev) for _ in range(3))
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        T.check(T.lib().tn_intersect(tree.handle, T.ptr(to), T.ptr(td), n, T.ptr(ids), T.ptr(r), T.ptr(s),
+                                     T.ptr(t), 0, torch.cuda.current_stream().cuda_stream))
+    for k in range(3):
+        for x in (ids, r, s, t):
+            x.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for x, y in zip((ids, r, s, t), ref):
+            assert torch.equal(x, y), f"replay {k}"
+    del g
+
+
 def test_render_tree_gives_the_same_hits(gpu):
     """TN_SCENE_EMPTY_LEAVES (rendering tree): the closest hit distance of
     every ray equals the reference tree's bit for bit; ids may differ only

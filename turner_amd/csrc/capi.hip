@@ -418,7 +418,10 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     std::unique_lock<std::mutex> work_lock;
     if (!(tick || leaf) && n >= slots * 256u) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        TN_HIP(hipStreamIsCapturing(stream, &cs));
+        if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
+            (void)hipGetLastError();  // e.g. the null stream while another stream captures
+            cs = hipStreamCaptureStatusActive;
+        }
         if (cs == hipStreamCaptureStatusNone) {
             work_lock = std::unique_lock<std::mutex>(sc->work_mu);
             wslot = (int)(sc->work_next++ % tn_scene::kWorkSlots);
