@@ -92,6 +92,31 @@ def test_icosphere_bit_exact(gpu, level):
         assert_bit_exact(gpu_hits(tree, o, d, sched | TN_ISECT_EXHAUSTIVE), ref, name + " exhaustive")
 
 
+def test_large_leaves_bit_exact(gpu):
+    """The reference tree of the occluded-light scene at level 7 keeps
+    clipped-away triangles in every later split: its leaves hold 28
+    references on average and up to 75, so the pooled pairs of one leaf can
+    fill more than a 64-pair round and most rounds continue a leaf of the
+    round before (owner marker at the round's first pair).  Random rays and
+    a bundle from one point, bit-exact against the oracle, cooperative and
+    per-lane schedules."""
+    sc = scenes.occluded_light_cornell(level=7)
+    tree = T.KDTree(sc["verts"])
+    lo, hi = np.asarray(tree.box()[:3]), np.asarray(tree.box()[3:])
+    rng = np.random.default_rng(17)
+    n = 1 << 14
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    # a bundle from one point: neighbouring lanes share leaves
+    o[: n // 2] = (0.5 * (lo + hi)).astype(np.float32)
+    print(f"[large leaves] {tree.num_triangles()} tris, {n} rays", flush=True)
+    ref = oracle_hits(tree, o, d)
+    for sched in (0, TN_ISECT_LEAF):
+        assert_bit_exact(gpu_hits(tree, o, d, sched), ref, f"large leaves sched {sched}")
+        assert_bit_exact(gpu_hits(tree, o, d, sched | TN_ISECT_EXHAUSTIVE), ref,
+                         f"large leaves sched {sched} exhaustive")
+
+
 def test_soup_and_grid_ties(gpu):
     rng = np.random.default_rng(11)
     soup = H.random_soup(3000, rng)
