@@ -231,22 +231,11 @@ def batch_throughput(args):
     import turner_amd as T
     from turner_amd import scenes
     from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_AUTOTUNE, TN_RENDER_COSINE, TN_RENDER_GRAPH
-    from turner_amd.batch import Batch
-    rng = np.random.default_rng(2024)
-    scs = [scenes.furnace(level=4, rho=0.8, radiance=1.0)]
-    levels = [4]
-    for k in range(1, 32):
-        lv = int(rng.integers(3, 9))
-        levels.append(lv)
-        scs.append(scenes.sphere_in_cornell(level=lv, seed=k))
+    from turner_amd.batch import Batch, heterogeneous_workload
+    scs, levels, jobs = heterogeneous_workload(seed=2024)
     t0 = time.perf_counter()
     trees = [T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True) for sc in scs]
     build_s = time.perf_counter() - t0
-    jobs = []
-    for sc in scs:
-        w = int(rng.integers(128, 1025))
-        jobs.append({"camera": sc["camera"], "lights": sc["lights"], "width": w, "height": w,
-                     "spp": int(rng.integers(16, 65))})
     b = Batch(trees)
     flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS | TN_RENDER_GRAPH | TN_RENDER_AUTOTUNE
     # tuning call (probes + one full render), then the timed render with the cached shape

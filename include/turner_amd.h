@@ -104,6 +104,12 @@ int tn_scene_get_triangle_records(const tn_scene* scene, float* out16, size_t ca
 int tn_kdtree_build(const float* verts9, uint32_t num_triangles, uint32_t flags,
                     uint64_t** flat_nodes, size_t* num_nodes, float* box6, uint32_t* height);
 void tn_buffer_free(void* p);
+/* Host-only: the triangle records tn_scene_create uploads (16 floats per
+ * triangle, as tn_scene_get_triangle_records returns them), i.e. the
+ * precompute of Triangle (lib/triangle.h:22-39): v0, the face normal
+ * normalize(cross(u, v)) with the cross product in double, u = v1 - v0,
+ * v = v2 - v0, then uv, vv, uu and denom = uv^2 - uu vv. */
+int tn_triangle_records_host(const float* verts9, uint32_t num_triangles, float* out16);
 /* Host-only: the device layout of the tree tn_scene_create would upload (with
  * flags TN_SCENE_SERIAL_BUILD / TN_SCENE_EMPTY_LEAVES): nodes2 holds two
  * uint32 per node -- inner (split bits, right << 2 | axis), leaf (offset into

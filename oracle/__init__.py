@@ -75,6 +75,9 @@ def lib():
         L.or_render.argtypes = [C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Light),
                                 C.c_uint32, C.POINTER(RenderParams), P, P,
                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.or_trace_rays.argtypes = [C.POINTER(Scene), C.POINTER(Light), C.c_uint32,
+                                    C.POINTER(RenderParams), P, P, C.c_uint32, C.c_int32, P,
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.or_exposure.argtypes = [C.c_float, C.c_float]
         L.or_exposure.restype = C.c_float
         L.or_gamma.argtypes = [C.c_float, C.c_float]
@@ -210,13 +213,7 @@ RENDERER_RAYCASTER = 1
 RENDERER_RAYTRACER = 2
 
 
-def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(0, 0, 0, 1),
-           nthreads=8, row_step=1, row_offset=0, want_sq=False, renderer=RENDERER_PATHTRACER,
-           max_visibility=2.0, shadow_intensity=0.5):
-    """Oracle render: returns (mean RGBA (h, w, 4), sum of squares or None, rays, shadow rays).
-    scene may hold "reflective" (N, 4) and "reflectivity" (N,) for the raytracer."""
-    keep = []
-
+def _scene_struct(scene, tree, keep):
     def arr(a, cols):
         a = _c(a).reshape(-1, cols)
         keep.append(a)
@@ -239,6 +236,11 @@ def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(
     for k, (p, col) in enumerate(lights):
         la[k].pos[:] = [float(x) for x in p]
         la[k].color[:] = [float(x) for x in col]
+    return sc, la, len(lights)
+
+
+def _params(width, height, max_depth, mc, spp, flags, seed, bg, nthreads, row_step, row_offset,
+            renderer, max_visibility, shadow_intensity):
     prm = RenderParams()
     prm.width, prm.height = width, height
     prm.max_depth, prm.mc_samples, prm.pixel_samples = max_depth, mc, spp
@@ -246,12 +248,40 @@ def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(
     prm.bg[:] = [float(x) for x in bg]
     prm.nthreads, prm.row_step, prm.row_offset = nthreads, row_step, row_offset
     prm.renderer, prm.max_visibility, prm.shadow_intensity = renderer, max_visibility, shadow_intensity
+    return prm
+
+
+def render(scene, tree, width, height, max_depth, mc, spp, flags=0, seed=1, bg=(0, 0, 0, 1),
+           nthreads=8, row_step=1, row_offset=0, want_sq=False, renderer=RENDERER_PATHTRACER,
+           max_visibility=2.0, shadow_intensity=0.5):
+    """Oracle render: returns (mean RGBA (h, w, 4), sum of squares or None, rays, shadow rays).
+    scene may hold "reflective" (N, 4) and "reflectivity" (N,) for the raytracer."""
+    keep = []
+    sc, la, nl = _scene_struct(scene, tree, keep)
+    prm = _params(width, height, max_depth, mc, spp, flags, seed, bg, nthreads, row_step, row_offset,
+                  renderer, max_visibility, shadow_intensity)
     out = np.zeros((height, width, 4), np.float32)
     sq = np.zeros((height, width, 4), np.float32) if want_sq else None
     nr, ns = C.c_uint64(0), C.c_uint64(0)
-    lib().or_render(C.byref(sc), C.byref(make_camera(scene["camera"])), la, len(lights),
+    lib().or_render(C.byref(sc), C.byref(make_camera(scene["camera"])), la, nl,
                     C.byref(prm), _p(out), _p(sq), C.byref(nr), C.byref(ns))
     return out, sq, nr.value, ns.value
+
+
+def trace_rays(scene, tree, o, d, depth=0, max_depth=3, mc=1, flags=0, seed=1, bg=(0, 0, 0, 1),
+               nthreads=8, renderer=RENDERER_PATHTRACER, max_visibility=2.0, shadow_intensity=0.5):
+    """trace() of trace.h:23-25 for caller rays entering at level `depth`:
+    returns (RGBA per ray (n, 4), closest-hit segments, shadow rays)."""
+    keep = []
+    sc, la, nl = _scene_struct(scene, tree, keep)
+    prm = _params(0, 0, max_depth, mc, 1, flags, seed, bg, nthreads, 1, 0, renderer, max_visibility,
+                  shadow_intensity)
+    o, d = _c(o).reshape(-1, 3), _c(d).reshape(-1, 3)
+    out = np.zeros((len(o), 4), np.float32)
+    nr, ns = C.c_uint64(0), C.c_uint64(0)
+    lib().or_trace_rays(C.byref(sc), la, nl, C.byref(prm), _p(o), _p(d), len(o), depth, _p(out),
+                        C.byref(nr), C.byref(ns))
+    return out, nr.value, ns.value
 
 
 def ppm(rgba):

@@ -1325,16 +1325,16 @@ static void* render_worker(void* arg) {
     return NULL;
 }
 
-void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
-               uint32_t nlights, const or_render_params* prm, float* out, float* out_sq,
-               uint64_t* n_rays, uint64_t* n_shadow) {
-    tracer T;
-    memset(&T, 0, sizeof(T));
-    T.sc = sc;
-    T.lights = lights;
-    T.nlights = nlights;
-    T.prm = prm;
-    /* emitter CDF */
+/* Emitter table of a scene (triangles with emission > 0): ids and the
+ * inclusive prefix of their areas, for area-light sampling.  Fills T and
+ * returns 0, or -1 when allocation fails. */
+static int tracer_setup(tracer* T, const or_scene* sc, const or_light* lights, uint32_t nlights,
+                        const or_render_params* prm) {
+    memset(T, 0, sizeof(*T));
+    T->sc = sc;
+    T->lights = lights;
+    T->nlights = nlights;
+    T->prm = prm;
     uint32_t ne = 0;
     for (uint32_t i = 0; i < sc->ntris; ++i) {
         const float* le = sc->emission4 ? sc->emission4 + 4 * (size_t)i : NULL;
@@ -1342,6 +1342,11 @@ void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
     }
     float* cdf = (float*)malloc((ne ? ne : 1) * sizeof(float));
     uint32_t* lid = (uint32_t*)malloc((ne ? ne : 1) * sizeof(uint32_t));
+    if (!cdf || !lid) {
+        free(cdf);
+        free(lid);
+        return -1;
+    }
     double acc = 0;
     uint32_t k = 0;
     for (uint32_t i = 0; i < sc->ntris && ne; ++i) {
@@ -1354,15 +1359,30 @@ void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
         lid[k] = i;
         k++;
     }
-    T.light_cdf = cdf;
-    T.light_ids = lid;
-    T.nemit = ne;
-    T.light_area = (float)acc;
+    T->light_cdf = cdf;
+    T->light_ids = lid;
+    T->nemit = ne;
+    T->light_area = (float)acc;
+    return 0;
+}
+
+static void tracer_release(tracer* T) {
+    free((void*)T->light_cdf);
+    free((void*)T->light_ids);
+}
+
+void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
+               uint32_t nlights, const or_render_params* prm, float* out, float* out_sq,
+               uint64_t* n_rays, uint64_t* n_shadow) {
+    tracer T;
+    memset(out, 0, sizeof(float) * 4 * (size_t)prm->width * prm->height);
+    if (out_sq) memset(out_sq, 0, sizeof(float) * 4 * (size_t)prm->width * prm->height);
+    if (n_rays) *n_rays = 0;
+    if (n_shadow) *n_shadow = 0;
+    if (tracer_setup(&T, sc, lights, nlights, prm)) return;
     uint32_t step = prm->row_step ? prm->row_step : 1;
     int single = (prm->flags & OR_FLAG_REF_RNG) && prm->nthreads <= 1;
     int nt = single ? 1 : (prm->nthreads < 1 ? 1 : prm->nthreads);
-    memset(out, 0, sizeof(float) * 4 * (size_t)prm->width * prm->height);
-    if (out_sq) memset(out_sq, 0, sizeof(float) * 4 * (size_t)prm->width * prm->height);
     render_job* jobs = (render_job*)calloc((size_t)nt, sizeof(render_job));
     pthread_t* th = (pthread_t*)calloc((size_t)nt, sizeof(pthread_t));
     for (int j = 0; j < nt; ++j) {
@@ -1386,8 +1406,75 @@ void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
     if (n_shadow) *n_shadow = ns;
     free(jobs);
     free(th);
-    free(cdf);
-    free(lid);
+    tracer_release(&T);
+}
+
+/* trace(ray, ..., depth, conf) (trace.h:23-25, pathtracer.cpp:14-102) for
+ * caller rays: ray j draws its own xorshift64* stream seeded from
+ * (seed, j).  Rays [i0, i1) of one worker. */
+typedef struct {
+    tracer proto;
+    const float *o3, *d3;
+    int32_t depth;
+    float* out4;
+    uint32_t i0, i1;
+    uint64_t nrays, nshadow;
+} trace_job;
+
+static void* trace_worker(void* arg) {
+    trace_job* J = (trace_job*)arg;
+    tracer T = J->proto;
+    T.cap = 256;
+    T.stk = (stack_item*)malloc(T.cap * sizeof(stack_item));
+    T.nrays = T.nshadow = 0;
+    for (uint32_t j = J->i0; j < J->i1; ++j) {
+        T.rng = splitmix64(T.prm->seed * 0x100000001B3ull + j) | 1;
+        v3 o = load3(J->o3 + 3 * (size_t)j), d = load3(J->d3 + 3 * (size_t)j);
+        col4 L;
+        if (T.prm->renderer == OR_RENDERER_RAYCASTER) L = raycast(&T, o, d);
+        else if (T.prm->renderer == OR_RENDERER_RAYTRACER) L = raytrace(&T, o, d, J->depth);
+        else L = trace(&T, o, d, J->depth);
+        float* q = J->out4 + 4 * (size_t)j;
+        q[0] = L.r; q[1] = L.g; q[2] = L.b; q[3] = L.a;
+    }
+    free(T.stk);
+    J->nrays = T.nrays;
+    J->nshadow = T.nshadow;
+    return NULL;
+}
+
+void or_trace_rays(const or_scene* sc, const or_light* lights, uint32_t nlights,
+                   const or_render_params* prm, const float* o3, const float* d3, uint32_t n,
+                   int32_t depth, float* out4, uint64_t* n_rays, uint64_t* n_shadow) {
+    tracer T;
+    if (n_rays) *n_rays = 0;
+    if (n_shadow) *n_shadow = 0;
+    if (n == 0 || tracer_setup(&T, sc, lights, nlights, prm)) return;
+    int nt = prm->nthreads < 1 ? 1 : prm->nthreads;
+    if ((uint32_t)nt > n) nt = (int)n;
+    trace_job* jobs = (trace_job*)calloc((size_t)nt, sizeof(trace_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nt, sizeof(pthread_t));
+    for (int j = 0; j < nt; ++j) {
+        jobs[j].proto = T;
+        jobs[j].o3 = o3;
+        jobs[j].d3 = d3;
+        jobs[j].depth = depth;
+        jobs[j].out4 = out4;
+        jobs[j].i0 = (uint32_t)((uint64_t)n * (uint64_t)j / (uint64_t)nt);
+        jobs[j].i1 = (uint32_t)((uint64_t)n * (uint64_t)(j + 1) / (uint64_t)nt);
+        pthread_create(&th[j], NULL, trace_worker, &jobs[j]);
+    }
+    uint64_t nr = 0, ns = 0;
+    for (int j = 0; j < nt; ++j) {
+        pthread_join(th[j], NULL);
+        nr += jobs[j].nrays;
+        ns += jobs[j].nshadow;
+    }
+    if (n_rays) *n_rays = nr;
+    if (n_shadow) *n_shadow = ns;
+    free(jobs);
+    free(th);
+    tracer_release(&T);
 }
 
 /* ---------------------------------------------------------------- output */

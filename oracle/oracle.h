@@ -12,8 +12,9 @@
  * reference's x86-64 build.
  *
  * Pinning (what checks each function against something independent):
- *   or_precompute        KATs of tests/test_triangle.cpp (unit normal, perpendicular)
- *                        + numpy float32 recomputation in tests
+ *   or_precompute        KATs of tests/test_triangle.cpp (unit normal, perpendicular),
+ *                        an FP64 recomputation (tests) and bit equality with the
+ *                        product's own precompute (tn_triangle_records_host)
  *   or_clip_triangle     KATs of tests/test_clipping.cpp:55-80
  *   or_ray_box           KATs of tests/test_intersection.cpp:15-36
  *   or_kdtree_build      KATs of tests/test_kdtree.cpp:16-63,162-186 (height,
@@ -23,9 +24,15 @@
  *                        over all triangles on random scenes + FP64 MT for r
  *   or_introsort         std::sort (libstdc++) on random inputs with ties (tests)
  *   or_xorshift64s       closed form of xorshift64* (Vigna 2016 / Marsaglia 2003)
- *   or_trace_ref         white-furnace closed form (rho * background) and
- *                        brute-force single-bounce integrals (tests)
- *   or_trace_ext         furnace closed form (rho * Le) and the unbiasedness
+ *   or_render / or_trace_rays (path tracer, pathtracer.cpp:14-102):
+ *                        white-furnace closed form (rho * background); the
+ *                        point-light term at depth 0 against rho max(0, l.n) Lc / pi
+ *                        with l from p, computed in FP64 (tests); the shadow
+ *                        predicate (lit iff no hit or r > |L - p2|) with blockers
+ *                        nearer than, exactly at and beyond the light; depth 1
+ *                        under a background c: rho c exactly with cosine sampling,
+ *                        in expectation with the reference's uniform sampling
+ *   extended estimator   furnace closed form (rho * Le) and the unbiasedness
  *                        relation cosine vs uniform sampling (tests)
  *   or_exposure/gamma    KATs of tests/test_effects.cpp
  *   or_ppm               KAT of tests/test_raster.cpp:44-57
@@ -141,6 +148,15 @@ typedef struct {
 void or_render(const or_scene* sc, const or_camera* cam, const or_light* lights,
                uint32_t nlights, const or_render_params* prm, float* out,
                float* out_sq, uint64_t* n_rays, uint64_t* n_shadow);
+
+/* trace(ray, tree, lights, depth, conf) (trace.h:23-25) for n caller rays:
+ * out4 (n x RGBA) receives each ray's radiance, entering at recursion level
+ * `depth`.  Ray j draws its own xorshift64* stream seeded from (seed, j).
+ * prm: max_depth, mc_samples, flags, seed, bg, nthreads, renderer and the
+ * plugin options are used (width, height, pixel samples and rows are not). */
+void or_trace_rays(const or_scene* sc, const or_light* lights, uint32_t nlights,
+                   const or_render_params* prm, const float* o3, const float* d3, uint32_t n,
+                   int32_t depth, float* out4, uint64_t* n_rays, uint64_t* n_shadow);
 
 /* Tone map (reference lib/effects.h:15-49) and PPM P3 (lib/raster.h:79-100). */
 float or_exposure(float light, float value);

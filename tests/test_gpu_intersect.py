@@ -279,7 +279,10 @@ def test_work_slots_cover_every_ray_once(gpu):
     sc = scenes.intersect_scene(level=5)
     tree = T.KDTree(sc["verts"])
     _, cus = T.device_info()
-    n = max(1 << 21, cus * 28 * 256 + 4321)  # above the threshold, ragged last chunk
+    # above the work-slot threshold (cus x 28 waves x 256 rays); the chunk is
+    # 64 rays there, and n is not a multiple of it, so the last chunk is ragged
+    n = max(1 << 21, cus * 28 * 256) + 4321
+    assert n % 64 != 0
     print(f"[slots] n={n}", flush=True)
     o, d = scenes.random_rays(n, seed=11)
     dev = torch.device("cuda:0")

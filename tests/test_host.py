@@ -164,3 +164,24 @@ def test_render_tree_covers_every_triangle(name):
             assert cells, (t, p)
             found = any(t in refs[nodes[c, 0]:nodes[c, 0] + (nodes[c, 1] >> 2)] for c in cells)
             assert found, f"triangle {t} point {p} not referenced by its leaf"
+
+
+def test_triangle_records_bitwise_equal_to_oracle():
+    """The product's precompute (the records tn_scene_create uploads) equals the
+    oracle's or_precompute bit for bit (lib/triangle.h:22-39): random soups
+    over several magnitudes, the reference test meshes, the displaced icosphere
+    and degenerate triangles (repeated vertices, collinear points)."""
+    from turner_amd import scenes
+    rng = np.random.default_rng(11)
+    sets = [rng.uniform(-1, 1, (200000, 9)), rng.uniform(-1e3, 1e3, (50000, 9)),
+            rng.uniform(-1e-3, 1e-3, (50000, 9)), H.cube12(), H.four_separated(), H.grid_mesh(8),
+            scenes.intersect_scene(level=5)["verts"]]
+    deg = rng.uniform(-1, 1, (1000, 9)).astype(np.float32)
+    deg[:500, 3:6] = deg[:500, 0:3]                      # v1 == v0
+    deg[500:, 6:9] = 2 * deg[500:, 3:6] - deg[500:, 0:3]  # v2 on the line v0 v1
+    sets.append(deg)
+    for v in sets:
+        v = np.ascontiguousarray(v, np.float32).reshape(-1, 9)
+        a = T.triangle_records(v)
+        b = O.precompute(v)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), len(v)

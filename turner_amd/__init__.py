@@ -42,6 +42,15 @@ def build_tree(verts, serial=False):
     return out, box, h.value
 
 
+def triangle_records(verts):
+    """Host precompute of the triangle records tn_scene_create uploads
+    (lib/triangle.h:22-39): (N, 16) float32, v0 n u v uv vv uu denom."""
+    v = _f32(verts, 9)
+    out = np.zeros((len(v), 16), np.float32)
+    check(lib().tn_triangle_records_host(ptr(v), len(v), ptr(out)))
+    return out
+
+
 def build_device_tree(verts, serial=False, empty_leaves=False):
     """Host-only: the device layout tn_scene_create would upload -- nodes
     (M, 2) uint32 (inner: split bits, right << 2 | axis; leaf: offset,
@@ -313,7 +322,9 @@ def render_accumulate(tree, cam, lights, conf, accum, sample_offset=0):
     """tn_render_accumulate: adds the per-pixel radiance SUMS of
     conf.num_pixel_samples samples (pixel-sample indices from sample_offset,
     fresh random streams per offset) into accum, a (h, w, 4) float32 device
-    tensor.  Asynchronous on torch's current stream; returns the stats."""
+    tensor.  Enqueued on torch's current stream; the call returns once the
+    render has finished (its stats and the queue-overflow check need the
+    device counters), so the sums are complete on return."""
     import torch
     w, h = conf.width, conf.height()
     if not (isinstance(accum, torch.Tensor) and accum.is_cuda and accum.dtype == torch.float32
@@ -327,6 +338,26 @@ def render_accumulate(tree, cam, lights, conf, accum, sample_offset=0):
                                      C.byref(p), ptr(accum), C.byref(st), stream))
     return {"rays": st.rays, "primary_rays": st.primary_rays, "shadow_rays": st.shadow_rays,
             "ms": st.ms}
+
+
+def trace(tree, o, d, lights, depth, conf, sample_offset=0):
+    """trace(ray, tree_intersection, lights, depth, conf) of trace.h:23-25 for
+    a batch of rays (tn_trace): o, d (n, 3) float32 host arrays; returns the
+    radiance of each ray entering at recursion level `depth` as (n, 4) RGBA
+    and the stats.  Ray j draws its streams from (conf.seed, sample_offset, j)."""
+    import torch
+    o, d = _f32(o, 3), _f32(d, 3)
+    n = len(o)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    rgba = torch.zeros(n * 4, device=dev)
+    p = _params(conf, 1, 1)
+    p.sample_offset = sample_offset
+    st = _lib.RenderStats()
+    check(lib().tn_trace(tree.handle, ptr(to), ptr(td), n, depth, _lights(lights), len(lights), C.byref(p),
+                         ptr(rgba), C.byref(st), torch.cuda.current_stream(dev).cuda_stream))
+    torch.cuda.current_stream(dev).synchronize()
+    return rgba.cpu().numpy().reshape(n, 4), {"rays": st.rays, "shadow_rays": st.shadow_rays}
 
 
 def write_pfm(path, rgb):

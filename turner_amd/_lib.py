@@ -116,6 +116,7 @@ SIGNATURES = {
     "tn_kdtree_build": (C.c_int, [_P, _U32, _U32, C.POINTER(C.POINTER(C.c_uint64)),
                                   C.POINTER(C.c_size_t), _P, C.POINTER(C.c_uint32)]),
     "tn_buffer_free": (None, [_P]),
+    "tn_triangle_records_host": (C.c_int, [_P, _U32, _P]),
     "tn_kdtree_build_device": (C.c_int, [_P, _U32, _U32, C.POINTER(C.POINTER(C.c_uint32)),
                                          C.POINTER(C.c_size_t), C.POINTER(C.POINTER(C.c_uint32)),
                                          C.POINTER(C.c_size_t), _P, C.POINTER(C.c_uint32)]),

@@ -11,7 +11,32 @@ import numpy as np
 from . import _lib
 from ._lib import TN_RENDER_AUTOTUNE, TN_RENDER_GRAPH, check, lib, ptr
 
-__all__ = ["Batch", "TN_RENDER_AUTOTUNE", "TN_RENDER_GRAPH"]
+__all__ = ["Batch", "heterogeneous_workload", "TN_RENDER_AUTOTUNE", "TN_RENDER_GRAPH"]
+
+
+def heterogeneous_workload(seed=2024, count=32):
+    """The batched-rendering workload of 32 independent scenes: scene 0 the
+    white furnace (unit-radiance enclosing emitter, level-4 sphere of albedo
+    0.8, whose pixels see exactly 0.8 where a camera ray hits the sphere);
+    scenes 1..count-1 seeded displaced icospheres of subdivision level
+    uniform in 3..8 (1,280 to 1.31 M triangles, log-uniform in the count) in
+    the Cornell enclosure with a 0.2 x 0.2 ceiling emitter of radiance 15;
+    square images of width uniform in 128..1024 and 16..64 pixel samples.
+    Returns (scenes, levels, jobs) with one job dict per scene."""
+    from . import scenes
+    rng = np.random.default_rng(seed)
+    scs = [scenes.furnace(level=4, rho=0.8, radiance=1.0)]
+    levels = [4]
+    for k in range(1, count):
+        lv = int(rng.integers(3, 9))
+        levels.append(lv)
+        scs.append(scenes.sphere_in_cornell(level=lv, seed=k))
+    jobs = []
+    for sc in scs:
+        w = int(rng.integers(128, 1025))
+        jobs.append({"camera": sc["camera"], "lights": sc["lights"], "width": w, "height": w,
+                     "spp": int(rng.integers(16, 65))})
+    return scs, levels, jobs
 
 
 class Batch:

@@ -90,7 +90,13 @@ def build_cli():
     for name, renderer in (("pathtracer", 0), ("raycaster", 1), ("raytracer", 2)):
         out = os.path.join(HERE, name)
         jobs.append(["g++", *inc, f"-DTN_CLI_RENDERER={renderer}", srcs[0], lib_obj, "-o", out, *link])
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    # the C++ caller test of the host interface (tests/test_gpu_host_api.py)
+    check_src = os.path.join(ROOT, "tests", "cpp", "host_api_check.cpp")
+    if os.path.exists(check_src):
+        jobs.append(["g++", *inc, check_src, lib_obj, "-o", os.path.join(ROOT, "tests", "cpp", "host_api_check"),
+                     f"-L{HERE}", "-lturner_amd", f"-L{os.path.join(ROCM, 'lib')}", "-lamdhip64",
+                     "-Wl,-rpath,$ORIGIN/../../turner_amd", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"])
+    with ThreadPoolExecutor(max_workers=4) as ex:
         list(ex.map(_run, jobs))
     return os.path.join(HERE, "pathtracer")
 

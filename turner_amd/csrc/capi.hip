@@ -276,6 +276,22 @@ int dev_upload(T** dst, const T* src, size_t count) {
     return TN_OK;
 }
 
+}  // namespace
+}  // namespace tn
+
+int tn_scene::reserve_stage(size_t bytes) {
+    if (d_stage && stage_cap >= bytes) return TN_OK;
+    if (d_stage) (void)hipFree(d_stage);
+    d_stage = nullptr;
+    stage_cap = 0;
+    TN_HIP(hipMalloc(&d_stage, bytes));
+    stage_cap = bytes;
+    return TN_OK;
+}
+
+namespace tn {
+namespace {
+
 void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_nodes);
     (void)hipFree(sc->d_refs);
@@ -291,6 +307,7 @@ void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_counters);
     (void)hipFree(sc->d_status);
     (void)hipFree(sc->d_work);
+    (void)hipFree(sc->d_stage);
     for (hipEvent_t& e : sc->work_ev)
         if (e) (void)hipEventDestroy(e);
 }
@@ -647,9 +664,10 @@ int tn_intersect_host(tn_scene* sc, const float* o3, const float* d3, uint32_t n
     if (n == 0) return TN_OK;
     if (!o3 || !d3 || !id || !r || !s || !t) return set_error(TN_ERR_INVALID, "NULL array");
     TN_HIP(hipSetDevice(sc->device));
-    char* buf = nullptr;
+    std::lock_guard<std::mutex> lock(sc->stage_mu);
     const size_t ray_bytes = (size_t)n * 12;
-    TN_HIP(hipMalloc(&buf, 2 * ray_bytes + (size_t)n * 16));
+    if (int rc = sc->reserve_stage(2 * ray_bytes + (size_t)n * 16)) return rc;
+    char* buf = sc->d_stage;
     float* d_o = (float*)buf;
     float* d_d = (float*)(buf + ray_bytes);
     uint32_t* d_id = (uint32_t*)(buf + 2 * ray_bytes);
@@ -668,7 +686,6 @@ int tn_intersect_host(tn_scene* sc, const float* o3, const float* d3, uint32_t n
             (e = hipMemcpy(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess)
             rc = hip_error(e, "hipMemcpy(hits)");
     }
-    (void)hipFree(buf);
     if (rc == TN_OK) {
         uint32_t st = 0;
         TN_HIP(hipMemcpy(&st, sc->d_status, 4, hipMemcpyDeviceToHost));

@@ -1,5 +1,5 @@
-// host_api.cpp -- host-only entry points of libturner_amd: kd-tree build
-// without a device and the hash-keyed tree cache.
+// host_api.cpp -- host-only entry points of libturner_amd: kd-tree build and
+// triangle precompute without a device, and the hash-keyed tree cache.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -100,6 +100,13 @@ int tn_kdtree_build_device(const float* verts9, uint32_t n, uint32_t flags, uint
 }
 
 void tn_buffer_free(void* p) { std::free(p); }
+
+int tn_triangle_records_host(const float* verts9, uint32_t n, float* out16) {
+    if (n && (!verts9 || !out16)) return tn::set_error(TN_ERR_INVALID, "NULL argument");
+    if (n > TN_MAX_TRIANGLES) return tn::set_error(TN_ERR_LIMIT, "too many triangles");
+    tn::precompute_triangles(verts9, n, out16);
+    return TN_OK;
+}
 
 int tn_tree_cache_save(const char* path, const float* verts9, uint32_t n, const uint64_t* nodes,
                        size_t num_nodes, const float* box6) {

@@ -201,6 +201,14 @@ __device__ __forceinline__ uint32_t rng_next(uint4& s) {
 }
 __device__ __forceinline__ float rng_uniform(uint4& s) { return (float)(rng_next(s) >> 8) * 0x1p-24f; }
 
+// Stream of child c: two words of the parent's stream, high word first (named
+// locals fix the order, which an expression with two calls would leave open).
+__device__ __forceinline__ uint4 child_seed(uint4& parent, uint32_t c) {
+    const uint64_t hi = rng_next(parent);
+    const uint64_t lo = rng_next(parent);
+    return rng_seed((hi << 32) | lo, c);
+}
+
 __device__ __forceinline__ float4 mul4(float4 a, float4 b) {
     return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
 }
@@ -831,7 +839,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                     Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
                     Q.d[ob][k] = make_float4(dir.x, dir.y, dir.z, pathbits);
                     Q.w[ob][k] = scale4(f, wr);
-                    Q.rng[ob][k] = rng_seed(((uint64_t)rng_next(rng) << 32) | rng_next(rng), c);
+                    Q.rng[ob][k] = child_seed(rng, c);
                 } else {
                     atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
                 }
@@ -881,7 +889,7 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
                     Q.o[ob][k] = make_float4(p2.x, p2.y, p2.z, __uint_as_float(pixel));
                     Q.d[ob][k] = make_float4(dir.x, dir.y, dir.z, pathbits);
                     Q.w[ob][k] = scale4(f, wr);
-                    Q.rng[ob][k] = rng_seed(((uint64_t)rng_next(rng) << 32) | rng_next(rng), c);
+                    Q.rng[ob][k] = child_seed(rng, c);
                 } else {
                     atomicOr(&Q.cnt[4 * kMaxLevels], 1u);
                 }
@@ -996,7 +1004,11 @@ int ensure_side_stream(RenderBuffers& rb) {
     return TN_OK;
 }
 
-std::mutex g_render_mutex;
+// Queues, streams and events of the renders on one device.  A render call
+// holds the context's lock from its first enqueued operation until its
+// streams have drained, so concurrent calls on one device take turns; the
+// buffers only grow (no allocation once a call of the same size has run).
+constexpr int kMaxDevices = 64;
 
 // Carves the queue buffers out of one allocation; with base == nullptr it
 // only measures (every piece is rounded up to 256 bytes).
@@ -1064,6 +1076,22 @@ int alloc_records(RecordBuffers& rb, size_t slots) {
     TN_HIP(hipMalloc(&rb.base, slots * 5 * sizeof(float4)));
     rb.slots = slots;
     return TN_OK;
+}
+
+struct RenderContext {
+    std::mutex mu;
+    RenderBuffers rbs[kPipes];
+    RecordBuffers recs[kPipes];
+    // main streams of pipelines 1.. (pipeline 0 runs on the caller's stream)
+    hipStream_t pipe_stream[kPipes] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kPipes] = {};
+    float* jitter = nullptr;  // reference pixel jitter (TN_RENDER_REF_JITTER)
+    size_t jitter_cap = 0;
+};
+
+RenderContext& render_context(int device) {
+    static RenderContext ctx[kMaxDevices];
+    return ctx[device];
 }
 
 // Pipeline p of n renders chunks p, p + n, p + 2n, ... of the call's samples.
@@ -1221,24 +1249,53 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     if (st) *st = tn_render_stats{};
     if (total == 0) return TN_OK;
 
-    std::lock_guard<std::mutex> lock(g_render_mutex);
-    static RenderBuffers rbs[kPipes];
-    // main streams of pipelines 1.. (pipeline 0 uses the caller's stream)
-    static hipStream_t pipe_stream[kPipes] = {};
-    static hipEvent_t ev_fork = nullptr, ev_join[kPipes] = {};
+    if (G.device < 0 || G.device >= kMaxDevices) return set_error(TN_ERR_LIMIT, "device %d", G.device);
+    RenderContext& C = render_context(G.device);
+    std::lock_guard<std::mutex> lock(C.mu);
+    RenderBuffers* rbs = C.rbs;
+    hipStream_t* pipe_stream = C.pipe_stream;
+    // Every exit after the first enqueued operation drains the streams this
+    // call used, so that no later call (once the lock is released) reuses
+    // queues or records that kernels of this one may still read.
+    bool enqueued = false;
+    auto fail = [&](int code) {
+        if (enqueued) {
+            (void)hipStreamSynchronize(stream);
+            for (int p = 0; p < kPipes; ++p) {
+                if (pipe_stream[p]) (void)hipStreamSynchronize(pipe_stream[p]);
+                if (rbs[p].side) (void)hipStreamSynchronize(rbs[p].side);
+            }
+        }
+        return code;
+    };
+#define TN_RHIP(call)                                                    \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) return fail(::tn::hip_error(e_, #call));   \
+    } while (0)
     for (int p = 0; p < npipes; ++p) {
         if ((rc = alloc_queues(rbs[p], cap, (uint32_t)table.size()))) return rc;
         if ((rc = ensure_side_stream(rbs[p]))) return rc;
     }
-    if (!ev_fork) {
-        TN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    if (!C.ev_fork) {
+        TN_HIP(hipEventCreateWithFlags(&C.ev_fork, hipEventDisableTiming));
         for (int p = 1; p < kPipes; ++p) {
             TN_HIP(hipStreamCreateWithFlags(&pipe_stream[p], hipStreamNonBlocking));
-            TN_HIP(hipEventCreateWithFlags(&ev_join[p], hipEventDisableTiming));
+            TN_HIP(hipEventCreateWithFlags(&C.ev_join[p], hipEventDisableTiming));
         }
     }
-    TN_HIP(hipMemcpyAsync(rbs[0].sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
-                          stream));
+    if (jitter_host && C.jitter_cap < jitter_len) {
+        if (C.jitter) (void)hipFree(C.jitter);
+        C.jitter = nullptr;
+        C.jitter_cap = 0;
+        TN_HIP(hipMalloc(&C.jitter, jitter_len * sizeof(float)));
+        C.jitter_cap = jitter_len;
+    }
+    for (int p = 0; p < npipes && training; ++p)
+        if ((rc = alloc_records(C.recs[p], (size_t)chunk * ((uint32_t)P->max_depth + 1)))) return rc;
+    enqueued = true;
+    TN_RHIP(hipMemcpyAsync(rbs[0].sp, table.data(), table.size() * sizeof(SceneParams), hipMemcpyHostToDevice,
+                           stream));
 
     RenderArgs A0{};
     A0.sv = G.sv;
@@ -1269,20 +1326,18 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         A0.rstride = (uint32_t)P->max_depth + 1;
         // every bounce of every path is a candidate; keep about kTrainSet of them
         constexpr uint32_t kTrainSet = 1u << 21;
-        if ((rc = guide_reserve_set(guide, kTrainSet, stream))) return rc;
+        if ((rc = guide_reserve_set(guide, kTrainSet, stream))) return fail(rc);
         const double candidates = (double)total * std::max(1, P->max_depth);
         A0.accept = (float)std::min(1.0, 0.95 * kTrainSet / std::max(1.0, candidates));
         A0.ts = guide->ts;
         A0.ts_count = guide->ts_count;
         A0.ts_cap = guide->ts_cap;
     }
-    float* d_jit = nullptr;
     if (jitter_host) {
-        TN_HIP(hipMalloc(&d_jit, jitter_len * sizeof(float)));
-        TN_HIP(hipMemcpyAsync(d_jit, jitter_host, jitter_len * sizeof(float), hipMemcpyHostToDevice, stream));
-        A0.jitter = d_jit;
+        TN_RHIP(hipMemcpyAsync(C.jitter, jitter_host, jitter_len * sizeof(float), hipMemcpyHostToDevice, stream));
+        A0.jitter = C.jitter;
     }
-    static RecordBuffers recs[kPipes];
+    RecordBuffers* recs = C.recs;
     RenderArgs A[kPipes];
     Queues Q[kPipes];
     for (int p = 0; p < npipes; ++p) {
@@ -1290,7 +1345,6 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         A[p].cs = rbs[p].cs;
         Q[p] = rbs[p].q;
         if (training) {
-            if ((rc = alloc_records(recs[p], (size_t)chunk * A0.rstride))) return rc;
             const size_t ns = recs[p].slots;
             A[p].rec_p = recs[p].base;
             A[p].rec_n = recs[p].base + ns;
@@ -1370,13 +1424,13 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     // graph with the same concurrency).
     auto round = [&](hipStream_t s0) -> hipError_t {
         if (npipes == 1) return chunk_sequence(0, s0);
-        hipError_t e = hipEventRecord(ev_fork, s0);
-        for (int p = 1; p < npipes && e == hipSuccess; ++p) e = hipStreamWaitEvent(pipe_stream[p], ev_fork, 0);
+        hipError_t e = hipEventRecord(C.ev_fork, s0);
+        for (int p = 1; p < npipes && e == hipSuccess; ++p) e = hipStreamWaitEvent(pipe_stream[p], C.ev_fork, 0);
         if (e == hipSuccess) e = chunk_sequence(0, s0);
         for (int p = 1; p < npipes && e == hipSuccess; ++p) e = chunk_sequence(p, pipe_stream[p]);
         for (int p = 1; p < npipes && e == hipSuccess; ++p) {
-            e = hipEventRecord(ev_join[p], pipe_stream[p]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s0, ev_join[p], 0);
+            e = hipEventRecord(C.ev_join[p], pipe_stream[p]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, C.ev_join[p], 0);
         }
         return e;
     };
@@ -1387,7 +1441,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         hipStream_t cap_stream = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        TN_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+        TN_RHIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             const hipError_t es = round(cap_stream);  // the other streams join and leave the capture
@@ -1400,32 +1454,26 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         (void)hipStreamDestroy(cap_stream);
-        if (e != hipSuccess) {
-            if (d_jit) (void)hipFree(d_jit);
-            return hip_error(e, "captured chunk sequence");
-        }
+        if (e != hipSuccess) return fail(hip_error(e, "captured chunk sequence"));
     } else {
         for (uint64_t c = 0; c < rounds; ++c) {
             const hipError_t e = round(stream);
-            if (e != hipSuccess) {
-                if (d_jit) (void)hipFree(d_jit);
-                return hip_error(e, "chunk sequence");
-            }
+            if (e != hipSuccess) return fail(hip_error(e, "chunk sequence"));
         }
     }
-    TN_HIP(hipGetLastError());
-    if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return rc;
+    TN_RHIP(hipGetLastError());
+    if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return fail(rc);
     unsigned long long tot[kPipes][2] = {};
     uint32_t overflow[kPipes] = {}, incomplete = 0;
     for (int p = 0; p < npipes; ++p) {
-        TN_HIP(hipMemcpyAsync(tot[p], Q[p].totals, 16, hipMemcpyDeviceToHost, stream));
-        TN_HIP(hipMemcpyAsync(&overflow[p], Q[p].cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
+        TN_RHIP(hipMemcpyAsync(tot[p], Q[p].totals, 16, hipMemcpyDeviceToHost, stream));
+        TN_RHIP(hipMemcpyAsync(&overflow[p], Q[p].cnt + 4 * kMaxLevels, 4, hipMemcpyDeviceToHost, stream));
     }
-    TN_HIP(hipMemcpyAsync(&incomplete, G.status + 2, 4, hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipStreamSynchronize(stream));
+    TN_RHIP(hipMemcpyAsync(&incomplete, G.status + 2, 4, hipMemcpyDeviceToHost, stream));
+    TN_RHIP(hipStreamSynchronize(stream));
+#undef TN_RHIP
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (d_jit) (void)hipFree(d_jit);
     for (int p = 0; p < npipes; ++p)
         if (overflow[p]) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
     if (incomplete) return set_error(TN_ERR_INCOMPLETE, "%u render traversals hit the visit cap", incomplete);
@@ -1691,8 +1739,9 @@ int tn_render(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32
         return set_error(TN_ERR_INVALID, "invalid render parameters");
     const size_t np = (size_t)P->width * P->height;
     TN_HIP(hipSetDevice(sc->device));
-    float* acc = nullptr;
-    TN_HIP(hipMalloc(&acc, np * 16 * 2));
+    std::lock_guard<std::mutex> lock(sc->stage_mu);
+    if (int rc = sc->reserve_stage(np * 16 * 2)) return rc;
+    float* acc = (float*)sc->d_stage;
     float* out = acc + np * 4;
     int rc = TN_OK;
     hipError_t e = hipMemsetAsync(acc, 0, np * 16, nullptr);
@@ -1712,7 +1761,7 @@ int tn_render(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint32
         if (!rc && (e = hipMemcpy(display, out, np * 16, hipMemcpyDeviceToHost)) != hipSuccess)
             rc = hip_error(e, "hipMemcpy");
     }
-    (void)hipFree(acc);
+    if (rc) (void)hipDeviceSynchronize();  // no kernel of this call still uses the staging memory
     return rc;
 }
 

@@ -49,6 +49,15 @@ struct tn_scene {
     bool work_ev_used[kWorkSlots] = {};
     uint32_t work_next = 0;
     std::mutex work_mu;
+    // Staging memory of the host-pointer calls (tn_intersect_host, tn_render):
+    // grows to the largest call and is reused, held under stage_mu for the
+    // whole (synchronous) call.
+    char* d_stage = nullptr;
+    size_t stage_cap = 0;
+    std::mutex stage_mu;
+
+    // Device memory of at least `bytes` in d_stage (caller holds stage_mu).
+    int reserve_stage(size_t bytes);
 
     tn::SceneView view() const {
         tn::SceneView v;

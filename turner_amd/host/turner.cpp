@@ -152,11 +152,41 @@ KDTree::~KDTree() {
     if (scene_) tn_scene_destroy(scene_);
 }
 
+namespace {
+
+// Root slab test of lib/intersection.h:105-128 on the direction with zero
+// components replaced by 1e-5 (fix_direction, lib/kdtree.cpp:503-511).
+bool enters_root_box(const Ray& ray, const float* box) {
+    const float o[3] = {ray.o.x, ray.o.y, ray.o.z};
+    float d[3] = {ray.d.x, ray.d.y, ray.d.z};
+    float tmin = 0, tmax = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (d[ax] == 0) d[ax] = 0.00001f;
+        const float inv = 1 / d[ax];
+        const float t1 = (box[ax] - o[ax]) * inv, t2 = (box[3 + ax] - o[ax]) * inv;
+        if (ax == 0) {
+            tmin = std::fmin(t1, t2);
+            tmax = std::fmax(t1, t2);
+        } else {
+            tmin = std::fmax(tmin, std::fmin(t1, t2));
+            tmax = std::fmin(tmax, std::fmax(t1, t2));
+        }
+    }
+    return !(tmax < tmin);
+}
+
+}  // namespace
+
 OptionalId KDTreeIntersection::intersect(const Ray& ray, float& r, float& a, float& b) {
     uint32_t id;
     float rr, ss, tt;
     check(tn_intersect_host(tree_->handle(), &ray.o.x, &ray.d.x, 1, &id, &rr, &ss, &tt, 0));
-    if (id == TN_INVALID_ID) return OptionalId{};
+    if (id == TN_INVALID_ID) {
+        // the reference sets r to FLT_MAX once the ray enters the root box
+        // (lib/kdtree.cpp:535) and leaves it there on a miss
+        if (enters_root_box(ray, tree_->box())) r = rr;
+        return OptionalId{};
+    }
     r = rr;
     a = ss;
     b = tt;
@@ -194,7 +224,13 @@ std::vector<Color> trace(const std::vector<Ray>& rays, KDTreeIntersection& ti,
     to_device(buf.p, od.data(), 24 * n);
     float* rgba = buf.as<float>() + 6 * n;
     if (hipMemset(rgba, 0, 16 * n) != hipSuccess) throw Error(TN_ERR_HIP, "hipMemset failed");
-    const tn_render_params p = params_of(conf, 1, 1);
+    tn_render_params p = params_of(conf, 1, 1);
+    // The reference draws hemisphere samples from one stream per worker
+    // thread that advances with every trace() call (lib/sampling.h:12); here
+    // each call of a thread takes the next range of tn_trace's streams, so
+    // repeated calls for the same ray give fresh samples.
+    static thread_local uint32_t calls = 0;
+    p.sample_offset = calls++;
     const auto ls = lights_of(lights);
     check(tn_trace(ti.tree().handle(), buf.as<float>(), buf.as<float>() + 3 * n, (uint32_t)n, depth,
                    ls.data(), (uint32_t)ls.size(), &p, rgba, nullptr, nullptr));

@@ -104,7 +104,8 @@ public:
     explicit KDTreeIntersection(const KDTree& tree) : tree_(&tree) {}
     const Triangle& operator[](uint32_t id) const { return (*tree_)[id]; }
     const Triangle& at(uint32_t id) const { return tree_->at(id); }
-    // One ray (lib/kdtree.cpp:515-578): r, a, b written on hit only.
+    // One ray (lib/kdtree.cpp:515-578): a and b are written on a hit only; r
+    // on a hit, and FLT_MAX on a miss of a ray that enters the root box.
     OptionalId intersect(const Ray& ray, float& r, float& a, float& b);
     OptionalId intersect(const Ray& ray) {
         float u;
