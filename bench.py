@@ -26,6 +26,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def host_cpu():
+    """Host CPU model, the logical CPUs this process may run on, and the thread
+    count the CPU baselines use: all of them, capped at OMP_NUM_THREADS when
+    it is set (the host's CPU share per GPU)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else avail
+    return {"model": model, "logical_cpus": avail, "nproc": os.cpu_count(), "threads": threads}
 METRIC = "Mrays/s + time-to-relMSE 0.01, guided vs plain tracer, 1 MI355X"
 
 
@@ -46,7 +68,8 @@ def main():
                     help="skip the batched heterogeneous rendering measurement")
     ap.add_argument("--guided-ref-spp", type=int, default=8192)
     ap.add_argument("--guided-width", type=int, default=512)
-    ap.add_argument("--render-spp", type=int, default=2)
+    ap.add_argument("--render-spp", type=int, default=16)
+    ap.add_argument("--render-repeats", type=int, default=3)
     ap.add_argument("--mlp-probe", action="store_true",
                     help="only the guide MLP kernels: 3 forward launches over 2^20 points and 3 "
                          "training steps of 2^16 samples (for rocprofv3 MFMA counters), then exit")
@@ -181,6 +204,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": ("HBM bytes per launch of this workload from rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                               "runs (calibrated on a 1 GiB copy), read from "
+                               "profiles/traffic_intersect_level8_2p24.txt; not measured in this run"
+                               if traffic is not None else None),
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel": "k_rays_coop<closest hit> (tn_intersect)",
         },
@@ -190,7 +217,8 @@ def main():
     if not args.no_cpu_baseline:
         import oracle as O
         m = 1 << args.cpu_sample_log2
-        ncores = min(os.cpu_count() or 1, 16)  # the box's CPU share for one GPU
+        cpu = host_cpu()
+        ncores = cpu["threads"]
         pre = tree.triangle_records()
         nodes = tree.flat_nodes()
         box = tree.box()
@@ -206,6 +234,7 @@ def main():
             "sample": (f"first 2^{args.cpu_sample_log2} of the 2^{args.rays_log2} rays, "
                        f"oracle restatement of KDTreeIntersection::intersect (exhaustive), "
                        f"{ncores} threads, {csec:.1f} s; GPU ids identical on the sample"),
+            "host": cpu,
         }
     if not args.no_render:
         out["render"] = render_throughput(args)
@@ -428,38 +457,98 @@ def render_throughput(args):
     """Plain wavefront path tracing at scale: the displaced level-8 icosphere
     (albedo 0.8) in the Cornell enclosure with a 0.2 x 0.2 emitter of
     radiance 15; 1920 x 1080, max-depth 3, monte-carlo-samples 4, cosine
-    sampling + area-light next-event estimation.  Runs --render-spp pixel
-    samples (the rate does not depend on the sample count)."""
+    sampling + area-light next-event estimation.  Times --render-spp pixel
+    samples per render, --render-repeats renders, with HIP events on the
+    render's stream, on the rendering kd-tree and on the reference kd-tree
+    (whose closest hits are the reference's bit for bit).  The CPU leg runs
+    the oracle's trace() over the reference tree on every 68th image row (16
+    rows, 2 pixel samples) on the host cores and extrapolates to the full
+    render."""
+    import types
+
+    import numpy as np
     import torch
 
     import turner_amd as T
     from turner_amd import scenes
     from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE
     sc = scenes.sphere_in_cornell(level=8)
-    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True)
     w, h = 1920, 1080
-    conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
-                          num_monte_carlo_samples=4, flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS, seed=1)
-    acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
-    T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)  # warm-up (queue allocation)
-    conf.num_pixel_samples = args.render_spp
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc, sample_offset=1)
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    cornell = cornell_gpu_vs_cpu(args)
-    plugins = plugin_throughput(args, sc, tree)
-    return {
-        "cornell_320": cornell,
-        "plugins": plugins,
-        "workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
-                     "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
-                     f"{args.render_spp} pixel samples timed, rendering kd-tree"),
-        "closest_hit_mrays_s": round(st["rays"] / sec / 1e6, 1),
-        "all_query_mrays_s": round((st["rays"] + st["shadow_rays"]) / sec / 1e6, 1),
-        "rays": st["rays"], "shadow_rays": st["shadow_rays"], "seconds": round(sec, 3),
-    }
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    stream = torch.cuda.current_stream()
+    out = {"workload": ("wavefront path tracing, level-8 icosphere (albedo 0.8) in the Cornell enclosure, "
+                        "0.2x0.2 emitter L=15, 1920x1080, max-depth 3, m 4, cosine + area-light NEE, "
+                        f"{args.render_spp} pixel samples per timed render, {args.render_repeats} renders")}
+    keep = None
+    ref_tree = None
+    for label, el in (("rendering_tree", True), ("reference_tree", False)):
+        t0 = time.perf_counter()
+        tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=el)
+        build_s = time.perf_counter() - t0
+        conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
+                              num_monte_carlo_samples=4, flags=flags, seed=1)
+        acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+        T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)  # warm-up (queue allocation)
+        conf.num_pixel_samples = args.render_spp
+        rates, walls, rays, shadow = [], [], 0, 0
+        for rep in range(args.render_repeats):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            w0 = time.perf_counter()
+            e0.record(stream)
+            st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc,
+                                     sample_offset=1 + rep * args.render_spp)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - w0)
+            ms = e0.elapsed_time(e1)
+            rates.append((st["rays"] / (ms * 1e-3) / 1e6, (st["rays"] + st["shadow_rays"]) / (ms * 1e-3) / 1e6, ms))
+            rays, shadow = st["rays"], st["shadow_rays"]
+        med = sorted(rates)[len(rates) // 2]
+        samples = w * h * args.render_spp
+        out[label] = {
+            "closest_hit_mrays_s": round(med[0], 1), "all_query_mrays_s": round(med[1], 1),
+            "closest_hit_mrays_s_each": [round(r[0], 1) for r in rates],
+            "ms_per_render": round(med[2], 2), "wall_s_each": [round(x, 3) for x in walls],
+            "rays_per_render": rays, "shadow_rays_per_render": shadow,
+            "full_128spp_render_s_extrapolated": round(med[2] * 1e-3 * 128 * w * h / samples, 2),
+            "host_build_s": round(build_s, 2), "device_nodes": tree.num_nodes(),
+            "hits": "bit-exact ids vs the reference" if not el else "same distances, ids may differ on exact ties",
+        }
+        if el:
+            keep = tree
+        else:
+            ref_tree = tree
+    # compatibility keys: the rendering-tree numbers
+    out["closest_hit_mrays_s"] = out["rendering_tree"]["closest_hit_mrays_s"]
+    out["all_query_mrays_s"] = out["rendering_tree"]["all_query_mrays_s"]
+    if not args.no_cpu_baseline:
+        import oracle as O
+        cpu = host_cpu()
+        rows, spp_c = 16, 2
+        step = h // rows + (1 if h % rows else 0)  # 68: rows 0, 68, ..., 1020
+        ot = types.SimpleNamespace(pre=ref_tree.triangle_records(), nodes=ref_tree.flat_nodes(), box=ref_tree.box())
+        c0 = time.perf_counter()
+        _, _, nr, ns = O.render(sc, ot, w, h, 3, 4, spp_c, flags=O.FLAG_COSINE | O.FLAG_AREA_LIGHTS, seed=3,
+                                row_step=step, row_offset=0, nthreads=min(cpu["threads"], rows))
+        csec = time.perf_counter() - c0
+        nrows = len(range(0, h, step))
+        scale = (h / nrows) * (128 / spp_c)
+        out["cpu_baseline"] = {
+            "rays_per_s": round(nr / csec, 1), "all_query_per_s": round((nr + ns) / csec, 1),
+            "cores": min(cpu["threads"], rows), "kind": "port", "host": cpu,
+            "sample": (f"rows 0, {step}, ... ({nrows} of {h}) at {spp_c} spp of the same render, oracle "
+                       f"restatement of trace() over the reference kd-tree (exhaustive traversal), "
+                       f"{csec:.1f} s; the full 1920x1080x128 render extrapolates linearly to "
+                       f"{csec * scale / 3600:.1f} h"),
+            "full_128spp_render_s_extrapolated": round(csec * scale, 1),
+        }
+        out["gpu_over_cpu_reference_tree"] = round(out["reference_tree"]["closest_hit_mrays_s"] * 1e6 /
+                                                   out["cpu_baseline"]["rays_per_s"], 1)
+    ref_tree.close()
+    out["cornell_320"] = cornell_gpu_vs_cpu(args)
+    out["plugins"] = plugin_throughput(args, sc, keep)
+    return out
 
 
 def plugin_throughput(args, sc, tree):
@@ -528,13 +617,14 @@ def cornell_gpu_vs_cpu(args):
     out = {"gpu_seconds": round(best, 4), "gpu_rays_per_s": round(st["rays"] / best, 1),
            "gpu_all_query_per_s": round((st["rays"] + st["shadow_rays"]) / best, 1)}
     if not args.no_cpu_baseline:
-        ncores = min(os.cpu_count() or 1, 16)
+        cpu = host_cpu()
+        ncores = cpu["threads"]
         ot = O.Tree(sc["verts"])
         t0 = time.perf_counter()
         _, _, nr, ns = O.render(sc, ot, 320, 320, 3, 1, 8, flags=O.FLAG_REF_RNG, nthreads=ncores)
         csec = time.perf_counter() - t0
         out["cpu_baseline"] = {"rays_per_s": round(nr / csec, 1), "all_query_per_s": round((nr + ns) / csec, 1),
-                               "cores": ncores, "kind": "port",
+                               "cores": ncores, "kind": "port", "host": cpu,
                                "sample": f"the full 320x320x8 render ({csec:.2f} s), oracle restatement of trace()"}
         out["gpu_over_cpu"] = round(out["gpu_rays_per_s"] / out["cpu_baseline"]["rays_per_s"], 1)
     return out

@@ -127,9 +127,14 @@ def test_trace_entering_at_depth_matches_oracle(gpu, depth, max_depth, mc, flags
     got, st = trace(sc, tree, o, d, depth, max_depth, mc, flags=flags)
     want, nr, ns = O.trace_rays(sc, otree, o, d, depth=depth, max_depth=max_depth, mc=mc, flags=flags, seed=7)
     zcheck(got, want, reps, what=f"depth {depth}/{max_depth} m {mc}")
-    # closed scene: every segment hits, so the segment count is n (1 + m + ... + m^levels-1)
+    # the box is open on the camera side, so the segment counts are random: a
+    # path adds between 1 and K = 1 + m + ... + m^(levels-1) segments, which
+    # bounds the spread of the count by sqrt(n) K / 2
     levels = max_depth - depth + 1
-    assert st["rays"] == len(o) * sum(mc**j for j in range(levels)) == nr
+    kmax = sum(mc**j for j in range(levels))
+    n = len(o)
+    assert n <= st["rays"] <= n * kmax and n <= nr <= n * kmax
+    assert abs(st["rays"] - nr) <= 5 * np.sqrt(2 * n) * kmax / 2
 
 
 # ---------------------------------------------------------------- C++ caller
