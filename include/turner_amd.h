@@ -156,6 +156,19 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
  */
 int tn_intersect(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
                  uint32_t* id, float* r, float* s, float* t, uint32_t flags, void* stream);
+/* tn_intersect for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): no reference counterpart.  Same arguments,
+ * results and flags as tn_intersect; allocates nothing and records no event.
+ * Large batches (more than 256 rays per resident wave) draw their chunks from
+ * one set of work counters held in a static device array (16 x 9 words per
+ * device, kept until the process ends), also under stream capture, where
+ * tn_intersect falls back to equal static shares.  The counters are cleared
+ * by a kernel at the start of every call, so calls on one device must not
+ * overlap: concurrent calls on different streams of one device race on the
+ * counters.  The scene's device must be current.  At most 2^31 rays per call
+ * (TN_ERR_LIMIT). */
+int tn_intersect_noalloc(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
+                         uint32_t* id, float* r, float* s, float* t, uint32_t flags, void* stream);
 /* Host-pointer variant (copies in and out, synchronous). */
 int tn_intersect_host(tn_scene* scene, const float* o3, const float* d3, uint32_t n,
                       uint32_t* id, float* r, float* s, float* t, uint32_t flags);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "tn_scene_get_flat_nodes": (C.c_int, [_P, _P, C.c_size_t]),
     "tn_scene_get_triangle_records": (C.c_int, [_P, _P, C.c_size_t]),
     "tn_intersect": (C.c_int, [_P, _P, _P, _U32, _P, _P, _P, _P, _U32, _P]),
+    "tn_intersect_noalloc": (C.c_int, [_P, _P, _P, _U32, _P, _P, _P, _P, _U32, _P]),
     "tn_intersect_host": (C.c_int, [_P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
     "tn_occluded": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32, _P]),
     "tn_get_counters": (C.c_int, [_P, _P, C.c_int]),

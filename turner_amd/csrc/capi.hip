@@ -412,7 +412,7 @@ int gather_leaf_records(const float4* tris, const uint32_t* refs, size_t nrefs, 
 
 int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
                 uint32_t* id, float* r, float* s, float* t, uint8_t* occ, uint32_t flags,
-                hipStream_t stream) {
+                hipStream_t stream, uint32_t* fixed_work) {
     if (n == 0) return TN_OK;
     if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
     const SceneView v = sc->view();
@@ -435,7 +435,13 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     uint32_t* work = nullptr;
     int wslot = -1;
     std::unique_lock<std::mutex> work_lock;
-    if (!(tick || leaf) && n >= slots * 256u) {
+    if (!(tick || leaf) && n >= slots * 256u && fixed_work) {
+        // the caller's counters (zeroed, not shared with a concurrent launch)
+        work = fixed_work;
+        const uint64_t per = ((uint64_t)n + 32ull * slots - 1) / (32ull * slots);
+        chunk = (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(64, (per + 63) & ~63ull));
+        waves = slots;
+    } else if (!(tick || leaf) && n >= slots * 256u) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
             (void)hipGetLastError();  // e.g. the null stream while another stream captures
@@ -462,7 +468,7 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     // after the launch: the slot is free again once this event has passed
     auto finish = [&]() -> int {
         TN_HIP(hipGetLastError());
-        if (work) {
+        if (wslot >= 0) {
             TN_HIP(hipEventRecord(sc->work_ev[wslot], stream));
             sc->work_ev_used[wslot] = true;
         }
@@ -518,6 +524,43 @@ int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n,
     return launch_rays(sc, o3, d3, nullptr, n, id, r, s, t, nullptr, flags, stream);
 }
 
+}  // namespace tn
+
+// Work counters of tn_intersect_noalloc: one work slot (8 per-XCD chunk
+// counters and a finished-wave counter, 16 words apart) in a device array, so
+// that the call allocates nothing.  Every device holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns with that device current (the array's name is a
+// host-side stand-in, not a device address).
+constexpr uint32_t kScratchWords = 16 * 9;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace tn {
+namespace {
+
+constexpr int kScratchDevices = 64;
+constexpr uint32_t kNoAllocMaxRays = 1u << 31;
+uint32_t* g_scratch[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device dev (current), looked up once per device.
+int scratch_address(int dev, uint32_t** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        TN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(TN_ERR_HIP, "no device address for the work counters");
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return TN_OK;
+}
+
+__global__ void k_clear_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+}  // namespace
 }  // namespace tn
 
 using namespace tn;
@@ -656,6 +699,27 @@ int tn_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uin
     if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
     if (n && (!o3 || !d3 || !id || !r || !s || !t)) return set_error(TN_ERR_INVALID, "NULL array");
     return launch_intersect(sc, o3, d3, n, id, r, s, t, flags, (hipStream_t)stream);
+}
+
+int tn_intersect_noalloc(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
+                         float* r, float* s, float* t, uint32_t flags, void* stream) {
+    if (!sc) return set_error(TN_ERR_INVALID, "scene is NULL");
+    if (n && (!o3 || !d3 || !id || !r || !s || !t)) return set_error(TN_ERR_INVALID, "NULL array");
+    if (n == 0) return TN_OK;
+    if (n > kNoAllocMaxRays) return set_error(TN_ERR_LIMIT, "%u rays exceed the limit of 2^31 per call", n);
+    int dev = -1;
+    TN_HIP(hipGetDevice(&dev));
+    if (dev != sc->device)
+        return set_error(TN_ERR_INVALID, "device %d is current, the scene lives on device %d", dev, sc->device);
+    if (dev < 0 || dev >= kScratchDevices) return set_error(TN_ERR_LIMIT, "device index %d", dev);
+    uint32_t* w = nullptr;
+    if (int rc = scratch_address(dev, &w)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // the counters start at zero (a kernel, not a memset, so that a captured
+    // graph replays it)
+    k_clear_words<<<1, 256, 0, st>>>(w, kScratchWords);
+    TN_HIP(hipGetLastError());
+    return launch_rays(sc, o3, d3, nullptr, n, id, r, s, t, nullptr, flags, st, w);
 }
 
 int tn_intersect_host(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,

@@ -110,9 +110,11 @@ __host__ __device__ inline uint32_t rays_per_wave(uint32_t n, uint32_t cus) {
 
 // Traversal over n rays (device arrays): closest hit into id/r/s/t, or with
 // occ != NULL visibility against tmax into occ.
+// fixed_work: zeroed work counters (16 x 9 words) the launch may use instead
+// of a work slot of the scene, also under stream capture (null: a slot).
 int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tmax, uint32_t n,
                 uint32_t* id, float* r, float* s, float* t, uint8_t* occ, uint32_t flags,
-                hipStream_t stream);
+                hipStream_t stream, uint32_t* fixed_work = nullptr);
 // Launches closest-hit traversal over n rays (device arrays) into id/r/s/t.
 int launch_intersect(tn_scene* sc, const float* o3, const float* d3, uint32_t n, uint32_t* id,
                      float* r, float* s, float* t, uint32_t flags, hipStream_t stream);
