@@ -90,7 +90,7 @@ def main():
 
     import turner_amd as T
     from turner_amd import scenes
-    from turner_amd._lib import TN_ISECT_COUNT
+    from turner_amd._lib import TN_ISECT_COUNT, TN_ISECT_REFINED
 
     if args.mlp_probe:
         print(json.dumps(mlp_probe()), flush=True)
@@ -102,7 +102,7 @@ def main():
     name, cus = T.device_info()
     sc = scenes.intersect_scene(level=args.level)
     t0 = time.perf_counter()
-    tree = T.KDTree(sc["verts"])
+    tree = T.KDTree(sc["verts"], refined=True)
     build_s = time.perf_counter() - t0
     n = 1 << args.rays_log2
     o, d = scenes.random_rays(n, seed=2024)
@@ -115,9 +115,49 @@ def main():
     stream = torch.cuda.current_stream(dev)
     L = T.lib()
 
-    def launch(flags=0):
+    def launch(flags=TN_ISECT_REFINED):
         T.check(L.tn_intersect(tree.handle, T.ptr(to), T.ptr(td), n, T.ptr(ids), T.ptr(r),
                                T.ptr(s), T.ptr(t), flags, stream.cuda_stream))
+
+    def counted(flags):
+        """algorithmic bytes of one launch and the per-ray work, from an
+        untimed counted launch: ray in (24) + hit out (16) + 8 per node
+        visited (inner + leaf) + 32 per test + 32 more for full-record tests
+        (the records are stored per triangle reference, so a test reads no
+        separate reference word)"""
+        cnt = np.zeros(8, np.uint64)
+        T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
+        launch(flags | TN_ISECT_COUNT)
+        T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
+        inner, leaves, refs, tests, rays, restarts, full = (int(x) for x in cnt[:7])
+        per_ray = {"inner_nodes": round(inner / rays, 2), "leaves": round(leaves / rays, 2),
+                   "triangle_tests": round(tests / rays, 2), "full_record_tests": round(full / rays, 2),
+                   "restarts": restarts}
+        return 40 * n + 8 * (inner + leaves) + 32 * tests + 32 * full, per_ray
+
+    def timed(flags, steps):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            launch(flags)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps, time.perf_counter() - w0
+
+    def traffic_of(fname):
+        """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+        runs (calibrated against a copy kernel), recorded with the profile
+        summary"""
+        tp = os.path.join(ROOT, "profiles", fname)
+        if os.path.exists(tp) and args.level == 8 and args.rays_log2 == 24:
+            with open(tp) as f:
+                for line in f:
+                    if line.startswith("hbm_bytes_per_launch:"):
+                        return int(line.split(":")[1])
+        return None
 
     if args.pmc_probe:
         launch()
@@ -129,45 +169,30 @@ def main():
         print(json.dumps({"pmc_probe": "done", "copy_bytes": src.numel() * 4}), flush=True)
         return
 
+    # the reference tree's own traversal: the closest hits every refined
+    # launch must reproduce bit for bit
+    launch(0)
+    ref_hits = [x.clone() for x in (ids, r, s, t)]
+    for _ in range(args.warmup):
+        launch(0)
+    ref_bytes, ref_per_ray = counted(0)
+    ref_ms, _ = timed(0, args.steps)
     for _ in range(args.warmup):
         launch()
-    # untimed counted launch: algorithmic bytes of the traversal
-    cnt = np.zeros(8, np.uint64)
-    T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
-    launch(TN_ISECT_COUNT)
-    T.check(L.tn_get_counters(tree.handle, cnt.ctypes.data, 1))
-    inner, leaves, refs, tests, rays, restarts, full = (int(x) for x in cnt[:7])
-    # bytes: ray in (24) + hit out (16) + 8 per node visited (inner + leaf)
-    # + 32 per test + 32 more for full-record tests (the records are stored
-    # per triangle reference, so a test reads no separate reference word)
-    alg_bytes = 40 * n + 8 * (inner + leaves) + 32 * tests + 32 * full
-
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    w0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        launch()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ms_per_step = e0.elapsed_time(e1) / args.steps
+    alg_bytes, per_ray = counted(TN_ISECT_REFINED)
+    ms_per_step, wall = timed(TN_ISECT_REFINED, args.steps)
     value = n / (ms_per_step * 1e-3) / 1e6
-
-    # sanity: the cube is closed, every ray must hit; a sample is bit-exact
+    # ids, r, s, t of the last timed launch == the reference traversal's, bit for bit
+    exact = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip((ids, r, s, t), ref_hits))
+    assert exact, "refined traversal differs from the reference tree's"
+    # the cube is closed: every ray must hit
     hid = ids.cpu().numpy().view(np.uint32)
     assert np.all(hid != T.TN_INVALID_ID), "a ray escaped the closed scene"
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
-    traffic = None
-    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    # (calibrated against a copy kernel), recorded with the profile summary
-    tp = os.path.join(ROOT, "profiles", "traffic_intersect_level8_2p24.txt")
-    if os.path.exists(tp) and args.level == 8 and args.rays_log2 == 24:
-        with open(tp) as f:
-            for line in f:
-                if line.startswith("hbm_bytes_per_launch:"):
-                    traffic = int(line.split(":")[1])
+    ref_achieved = ref_bytes / (ref_ms * 1e-3) / 1e9
+    traffic = traffic_of("traffic_intersect_refined_level8_2p24.txt")
+    ref_traffic = traffic_of("traffic_intersect_level8_2p24.txt")
+    info = tree.info
 
     out = {
         "metric": METRIC,
@@ -184,18 +209,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": ("ray-batch closest hit, level-%d displaced icosphere + 12-tri cube "
-                         "(%d triangles), reference SAH kd-tree, 2^%d rays, ids bit-exact"
+                         "(%d triangles), reference SAH kd-tree with its leaves refined into sub-trees "
+                         "(TN_SCENE_REFINED), 2^%d rays; ids, r, s, t of every timed launch equal to "
+                         "the reference tree's traversal bit for bit (checked on the full batch)"
                          % (args.level, tree.num_triangles(), args.rays_log2)),
             "triangles": tree.num_triangles(),
             "rays_per_step": n,
             "kdtree_nodes": tree.num_nodes(),
             "kdtree_height": tree.height(),
+            "refined_nodes": info.refined_nodes,
+            "refined_refs": info.refined_refs,
             "host_build_s": round(build_s, 2),
             "device": f"{name}, {cus} CUs",
-            "per_ray": {"inner_nodes": round(inner / rays, 2), "leaves": round(leaves / rays, 2),
-                        "triangle_tests": round(tests / rays, 2),
-                        "full_record_tests": round(full / rays, 2),
-                        "restarts": restarts},
+            "per_ray": per_ray,
+            "bit_exact_vs_reference_traversal": exact,
         },
         "roofline": {
             "bound": "hbm",
@@ -206,10 +233,18 @@ def main():
             "traffic": traffic,
             "traffic_source": ("HBM bytes per launch of this workload from rocprofv3 FETCH_SIZE and WRITE_SIZE "
                                "runs (calibrated on a 1 GiB copy), read from "
-                               "profiles/traffic_intersect_level8_2p24.txt; not measured in this run"
+                               "profiles/traffic_intersect_refined_level8_2p24.txt; not measured in this run"
                                if traffic is not None else None),
             "algorithmic_bytes_per_launch": alg_bytes,
-            "kernel": "k_rays_coop<closest hit> (tn_intersect)",
+            "kernel": "k_rays_coop<closest hit, refined tree> (tn_intersect, TN_ISECT_REFINED)",
+        },
+        "reference_tree_traversal": {
+            "value": round(n / (ref_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "ms_per_step": round(ref_ms, 4),
+            "per_ray": ref_per_ray,
+            "roofline": {"bound": "hbm", "achieved": round(ref_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ref_achieved / HBM_PEAK_GBS, 4), "traffic": ref_traffic,
+                         "algorithmic_bytes_per_launch": ref_bytes,
+                         "kernel": "k_rays_coop<closest hit> (tn_intersect, default flags)"},
         },
         "wall_s_timed_region": round(wall, 4),
     }
@@ -459,8 +494,9 @@ def render_throughput(args):
     radiance 15; 1920 x 1080, max-depth 3, monte-carlo-samples 4, cosine
     sampling + area-light next-event estimation.  Times --render-spp pixel
     samples per render, --render-repeats renders, with HIP events on the
-    render's stream, on the rendering kd-tree and on the reference kd-tree
-    (whose closest hits are the reference's bit for bit).  The CPU leg runs
+    render's stream, on the rendering kd-tree, on the refined reference tree
+    and on the reference kd-tree (the last two give the reference's closest
+    hits bit for bit).  The CPU leg runs
     the oracle's trace() over the reference tree on every 68th image row (16
     rows, 2 pixel samples) on the host cores and extrapolates to the full
     render."""
@@ -481,9 +517,16 @@ def render_throughput(args):
                         f"{args.render_spp} pixel samples per timed render, {args.render_repeats} renders")}
     keep = None
     ref_tree = None
-    for label, el in (("rendering_tree", True), ("reference_tree", False)):
+    hits = {"rendering_tree": "same distances, ids may differ on exact ties",
+            "refined_reference_tree": ("bit-exact ids vs the reference: the reference tree with its leaves refined "
+                                       "into sub-trees, exact ties and rays with a zero direction component "
+                                       "re-traced on the reference tree"),
+            "reference_tree": "bit-exact ids vs the reference"}
+    for label, kw in (("rendering_tree", {"empty_leaves": True}), ("refined_reference_tree", {"refined": True}),
+                      ("reference_tree", {})):
+        el = label == "rendering_tree"
         t0 = time.perf_counter()
-        tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=el)
+        tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], **kw)
         build_s = time.perf_counter() - t0
         conf = T.TracerConfig(width=w, aspect=w / h, max_recursion_depth=3, num_pixel_samples=1,
                               num_monte_carlo_samples=4, flags=flags, seed=1)
@@ -513,15 +556,17 @@ def render_throughput(args):
             "rays_per_render": rays, "shadow_rays_per_render": shadow,
             "full_128spp_render_s_extrapolated": round(med[2] * 1e-3 * 128 * w * h / samples, 2),
             "host_build_s": round(build_s, 2), "device_nodes": tree.num_nodes(),
-            "hits": "bit-exact ids vs the reference" if not el else "same distances, ids may differ on exact ties",
+            "hits": hits[label],
         }
         if el:
             keep = tree
-        else:
+        elif label == "reference_tree":
             ref_tree = tree
-    # compatibility keys: the rendering-tree numbers
-    out["closest_hit_mrays_s"] = out["rendering_tree"]["closest_hit_mrays_s"]
-    out["all_query_mrays_s"] = out["rendering_tree"]["all_query_mrays_s"]
+        else:
+            tree.close()
+    # compatibility keys: the bit-exact (refined reference tree) numbers
+    out["closest_hit_mrays_s"] = out["refined_reference_tree"]["closest_hit_mrays_s"]
+    out["all_query_mrays_s"] = out["refined_reference_tree"]["all_query_mrays_s"]
     if not args.no_cpu_baseline:
         import oracle as O
         cpu = host_cpu()

@@ -47,6 +47,8 @@ typedef struct {
     uint32_t num_emitters;  /* triangles with emission > 0 */
     float box[6];           /* KDTree::box  lib/kdtree.h:221 (min xyz, max xyz) */
     double build_ms;        /* host SAH build time */
+    uint32_t refined_nodes; /* TN_SCENE_REFINED: nodes of the refined tree (else 0) */
+    uint32_t refined_refs;  /* TN_SCENE_REFINED: triangle references of the refined tree */
 } tn_scene_info;
 
 /* Scene creation flags */
@@ -64,6 +66,19 @@ typedef struct {
  * order differs); the scene has no reference FlatNode array
  * (tn_scene_get_flat_nodes fails).  The reference tree stays the default. */
 #define TN_SCENE_EMPTY_LEAVES 2u
+/* Refined reference tree, for renders whose hit ids are the reference's.  The
+ * scene keeps the reference tree (tn_intersect, tn_scene_get_flat_nodes) and
+ * adds a refined copy of it: the same inner nodes, leaves reached in the same
+ * order, each leaf's references filtered to triangles whose bounding box
+ * reaches the leaf (the others were sunk into it by the reference's
+ * unbounded-box quirk) and larger leaves split further by rendering-variant
+ * sub-trees.  Renders (and tn_intersect with TN_ISECT_REFINED) traverse the
+ * refined tree, detect every ray whose closest hit ties with another triangle
+ * at exactly the same distance -- only there can the order inside a leaf
+ * decide the id -- and re-trace those rays, and rays with a zero direction
+ * component, on the reference tree.  Not combinable with
+ * TN_SCENE_EMPTY_LEAVES. */
+#define TN_SCENE_REFINED 4u
 
 /*
  * Replaces KDTree::KDTree(Triangles) (lib/kdtree.h:195, lib/kdtree.cpp:474-490):
@@ -111,7 +126,8 @@ void tn_buffer_free(void* p);
  * v = v2 - v0, then uv, vv, uu and denom = uv^2 - uu vv. */
 int tn_triangle_records_host(const float* verts9, uint32_t num_triangles, float* out16);
 /* Host-only: the device layout of the tree tn_scene_create would upload (with
- * flags TN_SCENE_SERIAL_BUILD / TN_SCENE_EMPTY_LEAVES): nodes2 holds two
+ * flags TN_SCENE_SERIAL_BUILD / TN_SCENE_EMPTY_LEAVES; with TN_SCENE_REFINED the
+ * refined tree): nodes2 holds two
  * uint32 per node -- inner (split bits, right << 2 | axis), leaf (offset into
  * refs, count << 2 | 3), left child = next node -- and refs the leaves'
  * triangle ids.  Free both with tn_buffer_free. */
@@ -138,6 +154,10 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
  * at a time).  Results are identical under every schedule. */
 #define TN_ISECT_TICK 8u       /* each lane: a few node steps or one triangle test per step */
 #define TN_ISECT_LEAF 16u      /* each lane: one leaf (all of its triangles) per step */
+/* TN_SCENE_REFINED scenes: traverse the refined tree; rays tied at their
+ * closest distance and rays with a zero direction component are re-traced on
+ * the reference tree, so results equal the default traversal's. */
+#define TN_ISECT_REFINED 32u
 /* Bits 24..27: occupancy variant of the default closest-hit kernel (waves
  * per SIMD 5..8, register budget accordingly; 0 = the tuned default). */
 #define TN_ISECT_WAVES(w) ((uint32_t)(w) << 24)

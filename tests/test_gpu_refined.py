@@ -1,0 +1,145 @@
+"""The refined reference tree (TN_SCENE_REFINED): traversals of it give the
+reference tree's results bit for bit (ids, r, s, t; occlusion), including
+exact ties (coplanar grids, the cube of the intersection benchmark) and rays
+with zero direction components, which are re-traced on the reference tree;
+renders over it equal renders over the reference tree (same hits, so the same
+paths and segment counts; pixels within FP32 accumulation order)."""
+import numpy as np
+import pytest
+
+import turner_amd as T
+from turner_amd import scenes
+from turner_amd._lib import (TN_ISECT_EXHAUSTIVE, TN_ISECT_REFINED, TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE,
+                             TN_RENDER_REF_JITTER)
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _isect(tree, to, td, flags):
+    import torch
+    n = to.shape[0]
+    ids = torch.empty(n, dtype=torch.int32, device="cuda")
+    r, s, t = (torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(3))
+    T.check(T.lib().tn_intersect(tree.handle, T.ptr(to), T.ptr(td), n, T.ptr(ids), T.ptr(r), T.ptr(s), T.ptr(t),
+                                 flags, torch.cuda.current_stream().cuda_stream))
+    return ids, r, s, t
+
+
+def _occ(tree, to, td, tmax, flags):
+    import torch
+    n = to.shape[0]
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    T.check(T.lib().tn_occluded(tree.handle, T.ptr(to), T.ptr(td), T.ptr(tmax), n, T.ptr(occ), flags,
+                                torch.cuda.current_stream().cuda_stream))
+    return occ
+
+
+def _same(tree, o, d, what):
+    """refined == default (ids, r, s, t) and occlusion at the hit distance, half of it and beyond"""
+    import torch
+    to, td = torch.from_numpy(np.ascontiguousarray(o, np.float32)).cuda(), \
+        torch.from_numpy(np.ascontiguousarray(d, np.float32)).cuda()
+    a = _isect(tree, to, td, 0)
+    b = _isect(tree, to, td, TN_ISECT_REFINED)
+    e = _isect(tree, to, td, TN_ISECT_REFINED | TN_ISECT_EXHAUSTIVE)
+    torch.cuda.synchronize()
+    for x, y, z in zip(a, b, e):
+        assert torch.equal(x, y), f"{what}: refined differs"
+        assert torch.equal(x, z), f"{what}: refined exhaustive differs"
+    r = a[1].clone()
+    for tm in (r, r * 0.5, torch.where(r < 1e30, r * 1.5, r)):
+        tm = tm.contiguous()
+        assert torch.equal(_occ(tree, to, td, tm, 0), _occ(tree, to, td, tm, TN_ISECT_REFINED)), what
+    hit = (a[0].cpu().numpy().view(np.uint32) != T.TN_INVALID_ID).mean()
+    print(f"[refined] {what}: {len(o)} rays, {hit:.3f} hit, bit-exact", flush=True)
+
+
+def test_refined_matches_reference_small_scenes(gpu):
+    rng = np.random.default_rng(5)
+    cases = [("grid (exact ties)", H.grid_mesh(24)), ("cube12", H.cube12()),
+             ("soup", H.random_soup(3000, rng)), ("icosphere level 5", scenes.intersect_scene(level=5)["verts"])]
+    for name, v in cases:
+        tree = T.KDTree(v, refined=True)
+        assert tree.info.refined_nodes > 0
+        o, d = H.hard_rays(v, 1 << 16, rng)  # a quarter axis-aligned: zero direction components
+        lo, hi = v.reshape(-1, 3).min(0), v.reshape(-1, 3).max(0)
+        o2 = rng.uniform(lo - 0.5, hi + 0.5, (1 << 16, 3))
+        d2 = rng.normal(size=(1 << 16, 3))
+        _same(tree, np.concatenate([o, o2]), np.concatenate([d, d2]), name)
+
+
+@pytest.mark.slow
+def test_refined_matches_reference_benchmark_batch(gpu):
+    """The intersection benchmark's full batch: 2^24 rays, 1.31 M triangles."""
+    sc = scenes.intersect_scene(level=8)
+    tree = T.KDTree(sc["verts"], refined=True)
+    print(f"[refined] reference nodes {tree.info.device_nodes} refs {tree.info.device_refs}, refined nodes "
+          f"{tree.info.refined_nodes} refs {tree.info.refined_refs}", flush=True)
+    o, d = scenes.random_rays(1 << 24, seed=2024)
+    _same(tree, o, d, "level-8 benchmark batch")
+
+
+@pytest.mark.slow
+def test_refined_matches_reference_render_batches(gpu):
+    """Render-like batches of the 1920x1080 scene (level-8 mesh in the Cornell
+    enclosure): camera rays, bounce rays from the camera hits, rays to the emitter."""
+    sc = scenes.sphere_in_cornell(level=8)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], refined=True)
+    rng = np.random.default_rng(9)
+    w, spp = 1024, 8
+    cam = sc["camera"]
+    n = w * w * spp
+    ys, xs = np.divmod(np.repeat(np.arange(w * w), spp), w)
+    xs = xs + rng.uniform(size=n)
+    ys = ys + rng.uniform(size=n)
+    cx = -cam["delta_x"] * (1 - 2 * xs / w)
+    cy = cam["delta_y"] * (1 - 2 * ys / w)
+    cd = (np.stack([cx, cy, -np.ones_like(cx)], -1) @ np.asarray(cam["trafo"]).T).astype(np.float32)
+    co = np.tile(np.asarray(cam["pos"], np.float32), (n, 1))
+    _same(tree, co, cd, "1920x1080 scene, camera rays")
+    ids, r, _, _ = T.KDTreeIntersection(tree).intersect_batch(co, cd)
+    hit = ids != T.TN_INVALID_ID
+    p = co[hit] + r[hit, None] * cd[hit]
+    v = sc["verts"].reshape(-1, 3, 3)[ids[hit]].astype(np.float64)
+    nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm *= -np.sign(np.sum(nrm * cd[hit], axis=1))[:, None]
+    p2 = (p + 1e-4 * nrm).astype(np.float32)
+    bd = rng.normal(size=p2.shape)
+    bd *= np.sign(np.sum(bd * nrm, axis=1))[:, None]
+    _same(tree, p2, bd.astype(np.float32), "1920x1080 scene, bounce rays")
+    em = sc["verts"].reshape(-1, 3, 3)[np.nonzero(sc["emission"][:, :3].sum(1) > 0)[0]]
+    k = rng.integers(0, len(em), len(p2))
+    su, u = np.sqrt(rng.uniform(size=len(p2))), rng.uniform(size=len(p2))
+    q = em[k, 0] * (1 - su)[:, None] + em[k, 1] * (su * (1 - u))[:, None] + em[k, 2] * (su * u)[:, None]
+    _same(tree, p2, (q - p2).astype(np.float32), "1920x1080 scene, rays to the emitter")
+
+
+def _render_sums(tree, sc, conf):
+    import torch
+    acc = torch.zeros((conf.height(), conf.width, 4), dtype=torch.float32, device="cuda")
+    st = T.render_accumulate(tree, sc["camera"], sc["lights"], conf, acc)
+    return acc.cpu().numpy(), st
+
+
+def test_refined_render_equals_reference_render(gpu):
+    """Same seeds, same hits: the same paths (equal segment and shadow-ray
+    counts) and the same pixel sums up to FP32 accumulation order, for the path
+    tracer (reference estimator with the point light; extended estimator with
+    the area emitter), the raycaster and the raytracer."""
+    for sc, flags, renderer in ((scenes.sphere_in_cornell(level=5), TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS, 0),
+                                (scenes.cornell_box(), TN_RENDER_REF_JITTER, 0),
+                                (scenes.sphere_in_cornell(level=5), 0, T.TN_RENDERER_RAYCASTER),
+                                (scenes.sphere_in_cornell(level=5), 0, T.TN_RENDERER_RAYTRACER)):
+        ref = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+        fin = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], refined=True)
+        lights = sc["lights"] or [((0.5, 0.9, 0.3), (1.0, 1.0, 1.0, 1.0))]
+        sc = dict(sc, lights=lights if renderer == T.TN_RENDERER_RAYTRACER else sc["lights"])
+        conf = T.TracerConfig(width=96, max_recursion_depth=3, num_pixel_samples=16, num_monte_carlo_samples=2,
+                              flags=flags, seed=5, renderer=renderer)
+        a, sa = _render_sums(ref, sc, conf)
+        b, sb = _render_sums(fin, sc, conf)
+        assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"]), (renderer, flags)
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5, err_msg=f"renderer {renderer} flags {flags}")
+        print(f"[refined render] renderer {renderer} flags {flags}: {sa['rays']} segments, equal", flush=True)

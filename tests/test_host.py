@@ -129,11 +129,12 @@ def _leaf_of_point(nodes, p):
 
 @pytest.mark.parametrize("name", ["soup", "grid", "sphere_walls"])
 def test_render_tree_covers_every_triangle(name):
-    """The rendering tree (TN_SCENE_EMPTY_LEAVES) drops fully clipped
-    triangles and keeps empty leaves; it must still be conservative: every
-    point of every triangle lies in a leaf that references the triangle, so a
-    ray hitting a triangle visits a leaf that tests it.  Also: no more
-    references than the reference tree, valid indices, one leaf per cell."""
+    """The rendering tree (TN_SCENE_EMPTY_LEAVES) drops triangles clipped far
+    away and keeps empty leaves; it must still be conservative: every point of
+    every triangle lies in a leaf that references the triangle, so a ray
+    hitting a triangle visits a leaf that tests it.  Also: valid indices, one
+    leaf per cell, and at most 4x the reference tree's references (its splits
+    lie between vertex coordinates, so straddling triangles are shared)."""
     rng = np.random.default_rng(5)
     if name == "soup":
         verts = H.random_soup(300, rng)
@@ -147,7 +148,7 @@ def test_render_tree_covers_every_triangle(name):
     nodes, refs, box, h = T.build_device_tree(verts, empty_leaves=True)
     rnodes, rrefs, rbox, _ = T.build_device_tree(verts)
     assert np.array_equal(box, rbox)
-    assert len(refs) <= len(rrefs)
+    assert len(refs) <= 4 * len(rrefs)
     inner = (nodes[:, 1] & 3) != 3
     assert np.all((nodes[inner, 1] >> 2) < len(nodes))
     leaf = ~inner
@@ -185,3 +186,43 @@ def test_triangle_records_bitwise_equal_to_oracle():
         a = T.triangle_records(v)
         b = O.precompute(v)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), len(v)
+
+
+def test_refined_tree_shape():
+    """TN_SCENE_REFINED on the host: the refined tree keeps every inner node of
+    the reference device tree in the same depth-first order, and every leaf
+    below a refined reference leaf holds only triangles of that leaf, among
+    them every triangle of the leaf whose bounding box meets the leaf's box."""
+    sc = scenes.occluded_light_cornell(level=4)
+    v = sc["verts"].reshape(-1, 3, 3)
+    rn, rr, rb, _ = T.build_device_tree(sc["verts"])
+    fn, fr, fb, _ = T.build_device_tree(sc["verts"], refined=True)
+    assert np.array_equal(rb, fb) and len(fr) < 2 * len(rr)
+    stack = [(0, 0, rb[:3].astype(np.float64), rb[3:].astype(np.float64))]
+    leaves = 0
+    while stack:
+        i, j, lo, hi = stack.pop()
+        x, y = int(rn[i, 0]), int(rn[i, 1])
+        fx, fy = int(fn[j, 0]), int(fn[j, 1])
+        if (y & 3) != 3:
+            assert (fx, fy & 3) == (x, y & 3), "an inner reference node changed"
+            ax, split = y & 3, float(np.array([x], np.uint32).view(np.float32)[0])
+            l_hi, r_lo = hi.copy(), lo.copy()
+            l_hi[ax] = split
+            r_lo[ax] = split
+            stack += [(y >> 2, fy >> 2, r_lo, hi), (i + 1, j + 1, lo, l_hi)]
+            continue
+        leaves += 1
+        leaf = set(rr[x:x + (y >> 2)].tolist())
+        below, sub = set(), [j]
+        while sub:
+            k = sub.pop()
+            kx, ky = int(fn[k, 0]), int(fn[k, 1])
+            if (ky & 3) == 3:
+                below |= set(fr[kx:kx + (ky >> 2)].tolist())
+            else:
+                sub += [ky >> 2, k + 1]
+        assert below <= leaf
+        touching = {t for t in leaf if np.all(v[t].min(0) <= hi) and np.all(v[t].max(0) >= lo)}
+        assert touching <= below, "a triangle meeting the leaf's box was dropped"
+    assert leaves > 100

@@ -51,7 +51,7 @@ def triangle_records(verts):
     return out
 
 
-def build_device_tree(verts, serial=False, empty_leaves=False):
+def build_device_tree(verts, serial=False, empty_leaves=False, refined=False):
     """Host-only: the device layout tn_scene_create would upload -- nodes
     (M, 2) uint32 (inner: split bits, right << 2 | axis; leaf: offset,
     count << 2 | 3), refs (K,) uint32, box (6,), height."""
@@ -60,7 +60,8 @@ def build_device_tree(verts, serial=False, empty_leaves=False):
     pr = C.POINTER(C.c_uint32)()
     nn, nr, h = C.c_size_t(), C.c_size_t(), C.c_uint32()
     box = np.zeros(6, np.float32)
-    flags = (_lib.TN_SCENE_SERIAL_BUILD if serial else 0) | (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0)
+    flags = (_lib.TN_SCENE_SERIAL_BUILD if serial else 0) | (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0) | \
+        (_lib.TN_SCENE_REFINED if refined else 0)
     check(lib().tn_kdtree_build_device(ptr(v), len(v), flags, C.byref(pn), C.byref(nn), C.byref(pr),
                                        C.byref(nr), ptr(box), C.byref(h)))
     try:
@@ -115,10 +116,12 @@ class KDTree:
     tree built earlier (e.g. from the tree cache) instead of building one.
     empty_leaves=True builds the rendering variant of the tree that keeps
     empty-space splits (TN_SCENE_EMPTY_LEAVES; no reference FlatNode array).
+    refined=True adds the refined reference tree that renders traverse with
+    the reference's hit ids (TN_SCENE_REFINED).
     """
 
     def __init__(self, verts, normals=None, albedo=None, emission=None, serial=False,
-                 flat_nodes=None, box=None, empty_leaves=False):
+                 flat_nodes=None, box=None, empty_leaves=False, refined=False):
         self._v = _f32(verts, 9)
         self._n = _f32(normals, 9)
         self._a = _f32(albedo, 4)
@@ -133,7 +136,7 @@ class KDTree:
                                                   C.byref(h)))
         else:
             flags = (_lib.TN_SCENE_SERIAL_BUILD if serial else 0) | \
-                (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0)
+                (_lib.TN_SCENE_EMPTY_LEAVES if empty_leaves else 0) | (_lib.TN_SCENE_REFINED if refined else 0)
             check(lib().tn_scene_create(ptr(self._v), ptr(self._n), ptr(self._a), ptr(self._e),
                                         n, flags, C.byref(h)))
         self.handle = h

@@ -23,11 +23,13 @@ TN_GUIDE_OUTPUTS = 40
 
 TN_SCENE_SERIAL_BUILD = 1
 TN_SCENE_EMPTY_LEAVES = 2
+TN_SCENE_REFINED = 4
 TN_ISECT_EXHAUSTIVE = 1
 TN_ISECT_MT = 2
 TN_ISECT_COUNT = 4
 TN_ISECT_TICK = 8
 TN_ISECT_LEAF = 16
+TN_ISECT_REFINED = 32
 TN_RENDER_COSINE = 1
 TN_RENDER_AREA_LIGHTS = 2
 TN_RENDER_REF_JITTER = 4
@@ -50,7 +52,8 @@ class SceneInfo(C.Structure):
     _fields_ = [("num_triangles", C.c_uint32), ("num_nodes", C.c_uint32),
                 ("height", C.c_uint32), ("device_nodes", C.c_uint32),
                 ("device_refs", C.c_uint32), ("num_emitters", C.c_uint32),
-                ("box", C.c_float * 6), ("build_ms", C.c_double)]
+                ("box", C.c_float * 6), ("build_ms", C.c_double),
+                ("refined_nodes", C.c_uint32), ("refined_refs", C.c_uint32)]
 
 
 class Camera(C.Structure):

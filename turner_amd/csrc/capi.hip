@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <type_traits>
 
 #include "tn_scene.h"
 
@@ -170,14 +171,19 @@ struct RayChunks {
 // Measured best on 2^24 rays against 1.31 M triangles, 512-ray chunks:
 // 5: 1048, 6: 1052, 7: 1143, 8: 1022 Mrays/s.
 constexpr int kCoopWavesPerEU = 7;
-template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU>
+// REF: traversal of the refined tree of a TN_SCENE_REFINED scene; a ray tied
+// at its closest distance (closest hit) or with a zero direction component is
+// marked for k_fix_rays (id kFixId, occ 2) instead of being stored.
+template <bool EXH, bool MT, bool ANY, bool COUNT, int WPE = kCoopWavesPerEU, bool REF = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rays_coop(
     SceneView sc, const float* __restrict__ o3, const float* __restrict__ d3,
     const float* __restrict__ tmax, uint32_t n, uint32_t chunk, uint32_t* __restrict__ out_id,
     float* __restrict__ out_r, float* __restrict__ out_s, float* __restrict__ out_t,
     uint8_t* __restrict__ occ, unsigned long long* counters, uint32_t* status, uint32_t* work) {
+    constexpr bool TIE = REF && !ANY;
+    using WL = std::conditional_t<TIE, CoopLdsTie, CoopLds>;
     __shared__ uint2 stk[kStack * kBlock];
-    __shared__ CoopLds wl[kBlock / 64];
+    __shared__ WL wl[kBlock / 64];
     __shared__ uint32_t red[8];  // [0..6] sums, [7] longest traversal in leaves
     if (COUNT && threadIdx.x < 8) red[threadIdx.x] = 0;
     if (COUNT) __syncthreads();
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t wave = (lblock * kBlock + threadIdx.x) >> 6;
     uint32_t nrays = 0;
     TraversalCounters cnt;
-    coop_loop<kStack, EXH, MT, ANY, COUNT>(
+    coop_loop<kStack, EXH, MT, ANY, COUNT, TIE>(
         sc, stk + threadIdx.x, kBlock, wl + threadIdx.x / 64,
         RayChunks{work, n, chunk, gridDim.x * (kBlock / 64), wave * chunk, blockIdx.x % kXcds, false},
         [&](uint32_t k, float3& o, float3& d, float& tm, uint32_t&) {
@@ -198,6 +204,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         [&](uint32_t k, const Trav& T, bool traversed) {
             const bool hit = traversed && T.best_id != kInvalidId;
             if (COUNT) atomicMax(&red[7], T.visits);  // leaves of the longest traversal
+            if (REF && (needs_reference(T.d) || (TIE && T.tie))) {
+                if (ANY) occ[k] = 2;
+                else out_id[k] = kFixId;
+                return;
+            }
             if (ANY) {
                 occ[k] = hit ? 1 : 0;
             } else {
@@ -221,6 +232,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         if (threadIdx.x < 7) atomicAdd(&counters[threadIdx.x], (unsigned long long)red[threadIdx.x]);
         if (threadIdx.x == 7) atomicMax(&counters[7], (unsigned long long)red[7]);
     }
+}
+
+// Rays k_rays_coop<..., REF> marked (id kFixId / occ 2), traversed one per
+// lane on the reference tree.
+template <bool EXH, bool ANY>
+__global__ __launch_bounds__(kBlock) void k_fix_rays(SceneView sc, const float* __restrict__ o3,
+                                                    const float* __restrict__ d3, const float* __restrict__ tmax,
+                                                    uint32_t n, uint32_t* __restrict__ out_id,
+                                                    float* __restrict__ out_r, float* __restrict__ out_s,
+                                                    float* __restrict__ out_t, uint8_t* __restrict__ occ,
+                                                    uint32_t* status) {
+    __shared__ uint2 stk[kStack * kBlock];
+    TraversalCounters cnt;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (ANY ? occ[k] != 2 : out_id[k] != kFixId) continue;
+        const float3 o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
+        const float3 d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
+        float r = FLT_MAX, s = 0.f, t = 0.f;
+        const uint32_t id = traverse<kStack, EXH, false, ANY, false>(sc, o, d, ANY ? tmax[k] : 0.f, r, s, t,
+                                                                   stk + threadIdx.x, kBlock, cnt);
+        const bool hit = id != kInvalidId;
+        if (ANY) {
+            occ[k] = hit ? 1 : 0;
+        } else {
+            out_id[k] = hit ? id : kInvalidId;
+            out_r[k] = hit ? r : FLT_MAX;
+            out_s[k] = hit ? s : 0.f;
+            out_t[k] = hit ? t : 0.f;
+        }
+    }
+    if (cnt.incomplete) atomicAdd(status, 1u);
 }
 
 __global__ void k_tonemap(const float4* __restrict__ acc, float4* __restrict__ out, uint32_t n,
@@ -297,6 +339,9 @@ void free_device(tn_scene* sc) {
     (void)hipFree(sc->d_refs);
     (void)hipFree(sc->d_tris);
     (void)hipFree(sc->d_lrec);
+    (void)hipFree(sc->d_fnodes);
+    (void)hipFree(sc->d_frefs);
+    (void)hipFree(sc->d_flrec);
     (void)hipFree(sc->d_normals);
     (void)hipFree(sc->d_albedo);
     (void)hipFree(sc->d_emission);
@@ -339,6 +384,15 @@ int finish_scene(tn_scene* sc, const float* verts9, const float* normals9, const
     sc->nrefs = (uint32_t)sc->tree.refs.size();
     if ((rc = dev_upload<float4>(&sc->d_lrec, nullptr, 4 * (size_t)sc->nrefs))) return rc;
     if ((rc = gather_leaf_records(sc->d_tris, sc->d_refs, sc->nrefs, sc->d_lrec))) return rc;
+    if (sc->refined) {
+        refine_reference_tree(verts9, sc->tree.nodes, sc->tree.refs, sc->tree.box, sc->parallel_build,
+                              sc->fnodes, sc->frefs, sc->fheight);
+        if ((rc = dev_upload(&sc->d_fnodes, reinterpret_cast<const uint2*>(sc->fnodes.data()), sc->fnodes.size())))
+            return rc;
+        if ((rc = dev_upload(&sc->d_frefs, sc->frefs.data(), sc->frefs.size()))) return rc;
+        if ((rc = dev_upload<float4>(&sc->d_flrec, nullptr, 4 * sc->frefs.size()))) return rc;
+        if ((rc = gather_leaf_records(sc->d_tris, sc->d_frefs, sc->frefs.size(), sc->d_flrec))) return rc;
+    }
     std::vector<float> nrm((size_t)n * 9);
     if (normals9) {
         std::memcpy(nrm.data(), normals9, nrm.size() * sizeof(float));
@@ -415,10 +469,13 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
                 hipStream_t stream, uint32_t* fixed_work) {
     if (n == 0) return TN_OK;
     if (n > (1u << 31)) return set_error(TN_ERR_LIMIT, "%u rays exceed 2^31 per call", n);
-    const SceneView v = sc->view();
     const bool exh = flags & TN_ISECT_EXHAUSTIVE, mt = flags & TN_ISECT_MT,
                cnt = flags & TN_ISECT_COUNT, any = occ != nullptr,
-               tick = flags & TN_ISECT_TICK, leaf = flags & TN_ISECT_LEAF;
+               tick = flags & TN_ISECT_TICK, leaf = flags & TN_ISECT_LEAF, refd = flags & TN_ISECT_REFINED;
+    if (refd && !sc->refined) return set_error(TN_ERR_INVALID, "TN_ISECT_REFINED needs a TN_SCENE_REFINED scene");
+    if (refd && (mt || tick || leaf || ((flags >> 24) & 15u)))
+        return set_error(TN_ERR_INVALID, "TN_ISECT_REFINED runs the cooperative reference leaf test only");
+    const SceneView v = refd ? sc->fview() : sc->view();
     // Rays per wave.  The cooperative kernel refills its lanes from one range
     // per wave, so every resident wave (7 per SIMD) gets an equal share of the
     // batch and all finish together (fixed 512-ray chunks left the last of
@@ -468,6 +525,18 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
     // after the launch: the slot is free again once this event has passed
     auto finish = [&]() -> int {
         TN_HIP(hipGetLastError());
+        if (refd) {  // marked rays on the reference tree
+            const SceneView ref = sc->view();
+            const uint32_t fb = std::min<uint32_t>((n + kBlock - 1) / kBlock, (uint32_t)sc->cus * 8u);
+            if (exh) {
+                if (any) k_fix_rays<true, true><<<fb, kBlock, 0, stream>>>(ref, o3, d3, tmax, n, id, r, s, t, occ, sc->d_status);
+                else k_fix_rays<true, false><<<fb, kBlock, 0, stream>>>(ref, o3, d3, tmax, n, id, r, s, t, occ, sc->d_status);
+            } else {
+                if (any) k_fix_rays<false, true><<<fb, kBlock, 0, stream>>>(ref, o3, d3, tmax, n, id, r, s, t, occ, sc->d_status);
+                else k_fix_rays<false, false><<<fb, kBlock, 0, stream>>>(ref, o3, d3, tmax, n, id, r, s, t, occ, sc->d_status);
+            }
+            TN_HIP(hipGetLastError());
+        }
         if (wslot >= 0) {
             TN_HIP(hipEventRecord(sc->work_ev[wslot], stream));
             sc->work_ev_used[wslot] = true;
@@ -491,7 +560,10 @@ int launch_rays(tn_scene* sc, const float* o3, const float* d3, const float* tma
         else go(k_rays<SC, false, false, A, C>);                         \
     }
 #define TN_SELC(A, C)                                                    \
-    if (exh) {                                                           \
+    if (refd) {                                                          \
+        if (exh) go(k_rays_coop<true, false, A, C, kCoopWavesPerEU, true>);  \
+        else go(k_rays_coop<false, false, A, C, kCoopWavesPerEU, true>);     \
+    } else if (exh) {                                                    \
         if (mt) go(k_rays_coop<true, true, A, C>);                       \
         else go(k_rays_coop<true, false, A, C>);                         \
     } else {                                                             \
@@ -588,8 +660,12 @@ int tn_scene_create(const float* verts9, const float* normals9, const float* alb
     *out = nullptr;
     if (!verts9 || n == 0) return set_error(TN_ERR_INVALID, "no triangles");
     if (n > TN_MAX_TRIANGLES) return set_error(TN_ERR_LIMIT, "%u triangles exceed 2^30-1", n);
+    if ((flags & TN_SCENE_REFINED) && (flags & TN_SCENE_EMPTY_LEAVES))
+        return set_error(TN_ERR_INVALID, "TN_SCENE_REFINED refines the reference tree (not with TN_SCENE_EMPTY_LEAVES)");
     std::unique_ptr<tn_scene> sc(new (std::nothrow) tn_scene);
     if (!sc) return set_error(TN_ERR_OOM, "host allocation failed");
+    sc->refined = (flags & TN_SCENE_REFINED) != 0;
+    sc->parallel_build = !(flags & TN_SCENE_SERIAL_BUILD);
     int rc = check_device(&sc->device);
     if (!rc) (void)hipDeviceGetAttribute(&sc->cus, hipDeviceAttributeMultiprocessorCount, sc->device);
     if (rc) return rc;
@@ -675,6 +751,8 @@ int tn_scene_get_info(const tn_scene* sc, tn_scene_info* info) {
     info->num_emitters = sc->nemit;
     std::memcpy(info->box, sc->tree.box, sizeof(info->box));
     info->build_ms = sc->tree.build_ms;
+    info->refined_nodes = (uint32_t)sc->fnodes.size();
+    info->refined_refs = (uint32_t)sc->frefs.size();
     return TN_OK;
 }
 

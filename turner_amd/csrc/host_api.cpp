@@ -69,8 +69,19 @@ int tn_kdtree_build_device(const float* verts9, uint32_t n, uint32_t flags, uint
         std::vector<uint32_t> r;
         uint32_t h = 0;
         const bool par = !(flags & TN_SCENE_SERIAL_BUILD);
-        if (flags & TN_SCENE_EMPTY_LEAVES) {
+        if ((flags & TN_SCENE_EMPTY_LEAVES) && (flags & TN_SCENE_REFINED)) {
+            return tn::set_error(TN_ERR_INVALID, "TN_SCENE_REFINED refines the reference tree");
+        } else if (flags & TN_SCENE_EMPTY_LEAVES) {
             tn::build_device_tree(verts9, n, par, nodes, r, box6, h);
+        } else if (flags & TN_SCENE_REFINED) {
+            std::vector<uint64_t> flat;
+            tn::build_kdtree(verts9, n, par, flat, box6);
+            std::string err;
+            std::vector<tn::DevNode> rn;
+            std::vector<uint32_t> rr;
+            if (!tn::flat_to_device(flat, n, rn, rr, err))
+                return tn::set_error(TN_ERR_INVALID, "invalid kd-tree: %s", err.c_str());
+            tn::refine_reference_tree(verts9, rn, rr, box6, par, nodes, r, h);
         } else {
             std::vector<uint64_t> flat;
             tn::build_kdtree(verts9, n, par, flat, box6);

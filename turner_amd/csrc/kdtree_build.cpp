@@ -11,6 +11,7 @@
 // instead of a heap vector per plane, and independent subtrees are built on
 // parallel host threads (the result does not depend on the thread count).
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <cfloat>
 #include <chrono>
@@ -66,11 +67,69 @@ inline void split(const Box& b, int ax, float pos, Box& l, Box& r) {
     l = box_of(b.lo, lmax);
     r = box_of(rmin, b.hi);
 }
+// b overlaps box enlarged by tol on every side
+inline bool near_box(const Box& b, const Box& box, float tol = kEps) {
+    return !(b.hi.x < box.lo.x - tol || b.hi.y < box.lo.y - tol || b.hi.z < box.lo.z - tol ||
+             box.hi.x + tol < b.lo.x || box.hi.y + tol < b.lo.y || box.hi.z + tol < b.lo.z);
+}
+// b clamped into box; an axis where b lies outside collapses onto box's nearer face
+inline Box clamp_to(const Box& b, const Box& box) {
+    Box r;
+    for (int ax = 0; ax < 3; ++ax) {
+        float lo = std::max(at(b.lo, ax), at(box.lo, ax)), hi = std::min(at(b.hi, ax), at(box.hi, ax));
+        if (hi < lo) {
+            const float face = at(b.hi, ax) < at(box.lo, ax) ? at(box.lo, ax) : at(box.hi, ax);
+            lo = hi = face;
+        }
+        set(r.lo, ax, lo);
+        set(r.hi, ax, hi);
+    }
+    return r;
+}
 inline float fmin3(float x, float y, float z) { return std::fmin(x, std::min(y, z)); }
 inline float fmax3(float x, float y, float z) { return std::fmax(x, std::max(y, z)); }
 inline Box tri_box(const float* t) {
     return box_of(mk(fmin3(t[0], t[3], t[6]), fmin3(t[1], t[4], t[7]), fmin3(t[2], t[5], t[8])),
                   mk(fmax3(t[0], t[3], t[6]), fmax3(t[1], t[4], t[7]), fmax3(t[2], t[5], t[8])));
+}
+
+// Triangle t meets box enlarged by tol on every side: the separating-axis
+// test of Akenine-Moeller, "Fast 3D triangle-box overlap testing" (2001) --
+// the box's three face normals, the triangle's normal and the nine cross
+// products of edges with the coordinate axes -- in FP64 with <= on touching,
+// so a degenerate triangle (zero edges or normal) reduces to its bounding box.
+inline bool tri_meets_box(const float* t, const Box& box, double tol) {
+    double c[3], h[3], v[3][3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const double lo = at(box.lo, ax), hi = at(box.hi, ax);
+        c[ax] = 0.5 * (lo + hi);
+        h[ax] = 0.5 * (hi - lo) + tol;
+        for (int k = 0; k < 3; ++k) v[k][ax] = (double)t[3 * k + ax] - c[ax];
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        const double lo = std::min(v[0][ax], std::min(v[1][ax], v[2][ax]));
+        const double hi = std::max(v[0][ax], std::max(v[1][ax], v[2][ax]));
+        if (lo > h[ax] || hi < -h[ax]) return false;
+    }
+    auto sep = [&](const double a[3]) {
+        double p0 = a[0] * v[0][0] + a[1] * v[0][1] + a[2] * v[0][2];
+        double p1 = a[0] * v[1][0] + a[1] * v[1][1] + a[2] * v[1][2];
+        double p2 = a[0] * v[2][0] + a[1] * v[2][1] + a[2] * v[2][2];
+        const double rad = h[0] * std::abs(a[0]) + h[1] * std::abs(a[1]) + h[2] * std::abs(a[2]);
+        return std::min(p0, std::min(p1, p2)) > rad || std::max(p0, std::max(p1, p2)) < -rad;
+    };
+    double e[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int ax = 0; ax < 3; ++ax) e[k][ax] = v[(k + 1) % 3][ax] - v[k][ax];
+    for (int k = 0; k < 3; ++k) {
+        const double a0[3] = {0.0, -e[k][2], e[k][1]};  // x cross e
+        const double a1[3] = {e[k][2], 0.0, -e[k][0]};  // y cross e
+        const double a2[3] = {-e[k][1], e[k][0], 0.0};  // z cross e
+        if (sep(a0) || sep(a1) || sep(a2)) return false;
+    }
+    const double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                         e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    return !sep(n);
 }
 
 // Thick-plane Sutherland-Hodgman against the 6 box planes (clipping.h:121-235,
@@ -173,22 +232,31 @@ inline float cost_(int trav, float lr, float rr, size_t nl, size_t nr) {
     return lambda_(nl, nr) * (trav + kIsect * (lr * nl + rr * nr));
 }
 
+// depth limit of the rendering variant (its trees are about 35 deep)
+constexpr int kMaxRenderDepth = 80;
+
 class Builder {
 public:
-    Builder(const float* verts, bool parallel, bool keep_empty = false)
-        : v_(verts), parallel_(parallel), keep_empty_(keep_empty) {
+    // keep_tol: how far beyond a node's box a triangle's bounding box may end
+    // and the triangle still be kept when clipping removes it (rendering variant)
+    Builder(const float* verts, bool parallel, bool keep_empty = false, float keep_tol = kEps)
+        : v_(verts), parallel_(parallel), keep_empty_(keep_empty), keep_tol_(keep_tol) {
         unsigned hc = std::thread::hardware_concurrency();
         budget_.store(parallel ? (int)(hc ? hc : 4) - 1 : 0);
     }
 
-    std::unique_ptr<Node> build(std::vector<uint32_t> tris, const Box& box) {
+    std::unique_ptr<Node> build(std::vector<uint32_t> tris, const Box& box, int depth = 0) {
         if (tris.empty()) return keep_empty_ ? leaf({}) : nullptr;
         if (tris.size() <= 3 || area(box) == 0) return leaf(std::move(tris));
+        // the rendering variant's conservative classification can give both
+        // children every triangle: such a split separates nothing
+        if (keep_empty_ && depth >= kMaxRenderDepth) return leaf(std::move(tris));
         std::vector<uint32_t> L, R;
         int ax;
         float pos;
         float min_cost = find_plane(tris, box, ax, pos, L, R);
         if (kIsect * tris.size() * lambda_(L.size(), R.size()) < min_cost) return leaf(std::move(tris));
+        if (keep_empty_ && L.size() == tris.size() && R.size() == tris.size()) return leaf(std::move(tris));
         Box lb, rb;
         split(box, ax, pos, lb, rb);
         tris.clear();
@@ -196,16 +264,16 @@ public:
         std::unique_ptr<Node> left, right;
         const bool fork = parallel_ && L.size() > 4096 && R.size() > 4096 && take_thread();
         if (fork) {
-            auto fut = std::async(std::launch::async, [this, &L, lb]() {
-                auto r = build(std::move(L), lb);
+            auto fut = std::async(std::launch::async, [this, &L, lb, depth]() {
+                auto r = build(std::move(L), lb, depth + 1);
                 budget_.fetch_add(1);
                 return r;
             });
-            right = build(std::move(R), rb);
+            right = build(std::move(R), rb, depth + 1);
             left = fut.get();
         } else {
-            left = build(std::move(L), lb);
-            right = build(std::move(R), rb);
+            left = build(std::move(L), lb, depth + 1);
+            right = build(std::move(R), rb, depth + 1);
         }
         if (!left) return right;
         if (!right) return left;
@@ -237,14 +305,47 @@ private:
                      float& out_pos, std::vector<uint32_t>& L, std::vector<uint32_t>& R) const {
         std::vector<Event> ev[3];
         for (auto& e : ev) e.reserve(2 * tris.size());
+        // rendering variant: the clipped box of every triangle taking part, for
+        // the conservative pass after the classification
+        std::vector<std::pair<uint32_t, Box>> kept;
+        if (keep_empty_) kept.reserve(tris.size());
         size_t num = 0;
+        // Rendering variant: in a node thinner than 4 kEps along some axis the
+        // thick-plane clip is unreliable (the tolerance zones of its two planes
+        // overlap, and the polygon can lose the part inside the node), so the
+        // triangle's bounding box clamped to the node stands in for it; such a
+        // node is not split again along its thin axes.
+        bool thin[3] = {false, false, false};
+        bool any_thin = false;
+        if (keep_empty_)
+            for (int a = 0; a < 3; ++a) any_thin |= (thin[a] = at(box.hi, a) - at(box.lo, a) < 4 * kEps);
         for (uint32_t id : tris) {
             bool gone = false;
-            Box cb = clip_triangle(v_ + 9 * (size_t)id, box, keep_empty_ ? &gone : nullptr);
-            // the reference keeps a fully clipped triangle with an unbounded box,
-            // so it follows every later split; the rendering variant drops it
-            if (gone || empty(cb)) continue;
+            Box cb;
+            if (any_thin) {
+                const Box tb = tri_box(v_ + 9 * (size_t)id);
+                if (!near_box(tb, box, keep_tol_)) continue;
+                cb = clamp_to(tb, box);
+            } else {
+                cb = clip_triangle(v_ + 9 * (size_t)id, box, keep_empty_ ? &gone : nullptr);
+            }
+            // The reference keeps a fully clipped triangle with an unbounded box,
+            // so it follows every later split.  The rendering variant drops it
+            // only when its bounding box lies beyond the clipping tolerance: the
+            // thick-plane clip also removes triangles that overlap the box in a
+            // sliver thinner than kEps, which a ray can still hit inside the box;
+            // those keep their bounding box clamped to this box.
+            if (gone) {
+                const Box tb = tri_box(v_ + 9 * (size_t)id);
+                if (!near_box(tb, box, keep_tol_)) continue;
+                cb = clamp_to(tb, box);
+            }
+            // the reference skips a triangle whose clipped box is a single point;
+            // the rendering variant keeps it: a ray aimed at that point (a vertex
+            // on the node's face) hits it inside this node
+            if (!keep_empty_ && empty(cb)) continue;
             ++num;
+            if (keep_empty_) kept.emplace_back(id, cb);
             for (int a = 0; a < 3; ++a) {
                 if (planar(cb, a)) {
                     ev[a].push_back(Event{id, at(cb.lo, a), at(cb.lo, a), kPlanar});
@@ -265,6 +366,7 @@ private:
         bool min_left = true;
         const float A = area(box);
         for (int a = 0; a < 3; ++a) {
+            if (thin[a]) continue;
             auto& E = ev[a];
             std::sort(E.begin(), E.end(), [](const Event& e1, const Event& e2) {
                 return e1.point < e2.point || (e1.point == e2.point && e1.type < e2.type);
@@ -279,19 +381,45 @@ private:
                 while (i < n && E[i].point == p && E[i].type == kStarting) { ++ps; ++i; }
                 np = pp;
                 nr -= pp + pe;
-                Box lb, rb;  // lib/kdtree.cpp:218-238
-                split(box, a, p, lb, rb);
-                float lr = area(lb) / A, rr = area(rb) / A;
-                const int trav = keep_empty_ ? kRenderTrav : kRefTrav;
-                float lc = cost_(trav, lr, rr, nl + np, nr);
-                float rc = cost_(trav, lr, rr, nl, np + nr);
-                float c = lc < rc ? lc : rc;
-                bool left = lc < rc;
-                if (c < min_cost) {
-                    min_cost = c;
-                    min_ax = a;
-                    min_pos = p;
-                    min_left = left;
+                if (keep_empty_) {
+                    // Rendering variant: the split goes midway between this
+                    // event position and the next, so no vertex lies on a split
+                    // plane (a ray aimed at such a vertex could reach it through
+                    // the child the triangle does not touch); everything that
+                    // started at or before p is left of it, everything not yet
+                    // ended right of it.  No split within 1/1024 of the node's
+                    // extent from its faces: such a child cuts next to nothing
+                    // off, yet the empty-side bonus would keep choosing it.
+                    if (i < n) {
+                        const float m = 0.5f * (p + E[i].point);
+                        const float margin = (at(box.hi, a) - at(box.lo, a)) * (1.0f / 1024.0f);
+                        if (p < m && m < E[i].point && m > at(box.lo, a) + margin && m < at(box.hi, a) - margin) {
+                            Box lb, rb;
+                            split(box, a, m, lb, rb);
+                            const float c = cost_(kRenderTrav, area(lb) / A, area(rb) / A, nl + pp + ps, nr);
+                            if (c < min_cost) {
+                                min_cost = c;
+                                min_ax = a;
+                                min_pos = m;
+                                min_left = true;
+                            }
+                        }
+                    }
+                } else {
+                    Box lb, rb;  // lib/kdtree.cpp:218-238
+                    split(box, a, p, lb, rb);
+                    float lr = area(lb) / A, rr = area(rb) / A;
+                    const int trav = keep_empty_ ? kRenderTrav : kRefTrav;
+                    float lc = cost_(trav, lr, rr, nl + np, nr);
+                    float rc = cost_(trav, lr, rr, nl, np + nr);
+                    float c = lc < rc ? lc : rc;
+                    bool left = lc < rc;
+                    if (c < min_cost) {
+                        min_cost = c;
+                        min_ax = a;
+                        min_pos = p;
+                        min_left = left;
+                    }
                 }
                 nl += ps + pp;
                 np = 0;
@@ -314,6 +442,24 @@ private:
                 R.push_back(e.id);
             }
         }
+        if (keep_empty_) {
+            // Conservative assignment: every triangle whose clipped box
+            // reaches the split plane (within a few ulps of its position)
+            // goes into both children.  The events alone miss a clipped box
+            // thinner than kEps, which counts as planar and goes by its low
+            // coordinate to one side even when it straddles the split (deep in
+            // a rendering tree nodes are thinner than kEps), and a vertex that
+            // rounding leaves exactly on the plane.
+            const float tol = std::ldexp(std::abs(min_pos), -21);
+            std::vector<uint32_t> sl(L), sr(R);
+            std::sort(sl.begin(), sl.end());
+            std::sort(sr.begin(), sr.end());
+            for (const auto& [id, cb] : kept) {
+                if (at(cb.lo, min_ax) > min_pos + tol || at(cb.hi, min_ax) < min_pos - tol) continue;
+                if (!std::binary_search(sl.begin(), sl.end(), id)) L.push_back(id);
+                if (!std::binary_search(sr.begin(), sr.end(), id)) R.push_back(id);
+            }
+        }
         out_ax = min_ax;
         out_pos = min_pos;
         return min_cost;
@@ -322,6 +468,7 @@ private:
     const float* v_;
     bool parallel_;
     bool keep_empty_;  // empty children become empty leaves instead of being dropped
+    float keep_tol_;
     std::atomic<int> budget_{0};
 };
 
@@ -414,11 +561,13 @@ void build_kdtree(const float* verts9, uint32_t n, bool parallel, std::vector<ui
 }
 
 // The same SAH build with two changes for rendering: a triangle that clipping
-// removes from a node entirely is dropped (the reference gives it an
-// unbounded box, lib/clipping.h:210-212, so it sinks into every descendant
-// and fills leaves with triangles that do not touch them), and a split whose
-// one side is empty keeps that side as an empty leaf (the reference drops it,
-// lib/kdtree.cpp:166-171).  Flattened straight into the device layout.
+// removes from a node entirely is dropped unless its bounding box reaches
+// within the clipping tolerance of the node (the reference gives every such
+// triangle an unbounded box, lib/clipping.h:210-212, so it sinks into every
+// descendant and fills leaves with triangles that do not touch them), and a
+// split whose one side is empty keeps that side as an empty leaf (the
+// reference drops it, lib/kdtree.cpp:166-171).  Flattened straight into the
+// device layout.
 void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vector<DevNode>& nodes,
                        std::vector<uint32_t>& refs, float box6[6], uint32_t& height) {
     std::vector<uint32_t> ids(n, 0);
@@ -470,6 +619,158 @@ void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vect
     }
     box6[0] = box.lo.x; box6[1] = box.lo.y; box6[2] = box.lo.z;
     box6[3] = box.hi.x; box6[4] = box.hi.y; box6[5] = box.hi.z;
+}
+
+// The reference tree with refined leaves.  Every inner node of the reference
+// device tree is kept, so leaves are reached in the reference's order; each
+// leaf keeps only the references whose triangle's bounding box reaches within
+// tol of the leaf's box (tol = kEps + 2^-16 of the box's largest coordinate
+// magnitude, far above the rounding of any hit point or split crossing: the
+// others are triangles the reference's unbounded-box quirk sank into the
+// leaf from an ancestor), and a leaf keeping more than four is replaced by a
+// rendering-variant sub-tree over its box (SAH node step 60, clipped-away
+// triangles kept within tol).  Closest hits then equal the reference tree's
+// except where two triangles are hit at exactly the same distance (the
+// winner depends on the order inside a leaf) or a direction component is
+// zero (its 1e-5 substitute bends the traversal path away from the ray):
+// the caller re-traces those rays on the reference tree.
+void refine_reference_tree(const float* verts9, const std::vector<DevNode>& rnodes,
+                           const std::vector<uint32_t>& rrefs, const float box6[6], bool parallel,
+                           std::vector<DevNode>& nodes, std::vector<uint32_t>& refs, uint32_t& height) {
+    struct LeafJob {
+        uint32_t node;
+        Box box;
+        std::vector<uint32_t> ids;
+        std::unique_ptr<Node> sub;
+    };
+    // leaves of the reference tree with their boxes
+    std::vector<LeafJob> jobs;
+    std::vector<uint32_t> job_of(rnodes.size(), 0xFFFFFFFFu);
+    {
+        struct It {
+            uint32_t node;
+            Box box;
+        };
+        std::vector<It> st{{0u, Box{mk(box6[0], box6[1], box6[2]), mk(box6[3], box6[4], box6[5])}}};
+        while (!st.empty()) {
+            It it = st.back();
+            st.pop_back();
+            const DevNode d = rnodes[it.node];
+            if ((d.y & 3u) == 3u) {
+                job_of[it.node] = (uint32_t)jobs.size();
+                LeafJob j;
+                j.node = it.node;
+                j.box = it.box;
+                j.ids.assign(rrefs.begin() + d.x, rrefs.begin() + d.x + (d.y >> 2));
+                jobs.push_back(std::move(j));
+                continue;
+            }
+            const int ax = (int)(d.y & 3u);
+            float pos;
+            std::memcpy(&pos, &d.x, 4);
+            Box l, r;
+            split(it.box, ax, pos, l, r);
+            st.push_back({d.y >> 2, r});
+            st.push_back({it.node + 1, l});
+        }
+    }
+    // Every leaf of a sub-tree gets every triangle of the reference leaf that
+    // meets the leaf's cell enlarged by tol, besides the builder's own choice
+    // (whose thick-plane clipping can drop a sliver beyond a split plane):
+    // a hit point inside a cell is then found in that cell.  Leaf order:
+    // ascending triangle id (exact ties are settled on the reference tree).
+    std::function<void(Node*, const Box&, const std::vector<uint32_t>&, float)> cover =
+        [&](Node* n, const Box& box, const std::vector<uint32_t>& cand, float tol) {
+            std::vector<uint32_t> mine;
+            mine.reserve(cand.size());
+            for (uint32_t id : cand)
+                if (tri_meets_box(verts9 + 9 * (size_t)id, box, tol)) mine.push_back(id);
+            if (!n->leaf) {
+                Box l, r;
+                split(box, n->axis, n->split, l, r);
+                cover(n->left.get(), l, mine, tol);
+                cover(n->right.get(), r, mine, tol);
+                return;
+            }
+            n->ids.insert(n->ids.end(), mine.begin(), mine.end());
+            std::sort(n->ids.begin(), n->ids.end());
+            n->ids.erase(std::unique(n->ids.begin(), n->ids.end()), n->ids.end());
+        };
+    auto refine = [&](LeafJob& j) {
+        float mag = 0.f;
+        for (int ax = 0; ax < 3; ++ax)
+            mag = std::max(mag, std::max(std::abs(at(j.box.lo, ax)), std::abs(at(j.box.hi, ax))));
+        const float tol = kEps + std::ldexp(mag, -16);
+        std::vector<uint32_t> keep;
+        keep.reserve(j.ids.size());
+        for (uint32_t id : j.ids)
+            if (near_box(tri_box(verts9 + 9 * (size_t)id), j.box, tol)) keep.push_back(id);
+        if (keep.size() > 4) {
+            Builder B(verts9, false, true, tol);
+            j.sub = B.build(keep, j.box);
+            cover(j.sub.get(), j.box, keep, tol);
+            j.ids.clear();
+        } else {
+            j.ids = std::move(keep);
+        }
+    };
+    const unsigned hc = parallel ? std::max(1u, std::thread::hardware_concurrency()) : 1u;
+    if (hc > 1 && jobs.size() > 64) {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < hc; ++t)
+            th.emplace_back([&]() {
+                for (size_t k; (k = next.fetch_add(1)) < jobs.size();) refine(jobs[k]);
+            });
+        for (auto& t : th) t.join();
+    } else {
+        for (auto& j : jobs) refine(j);
+    }
+    // depth-first emission: reference inner nodes, then each leaf's sub-tree
+    nodes.clear();
+    refs.clear();
+    height = 0;
+    struct Item {
+        const Node* sub;    // a sub-tree node, or null for a reference node
+        uint32_t rnode;     // the reference node (when sub is null)
+        uint32_t parent;    // node whose right index this one is, or ~0
+        uint32_t level;
+    };
+    std::vector<Item> st{{nullptr, 0u, 0xFFFFFFFFu, 0u}};
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        const uint32_t me = (uint32_t)nodes.size();
+        if (it.parent != 0xFFFFFFFFu) nodes[it.parent].y |= me << 2;
+        height = std::max(height, it.level);
+        const Node* n = it.sub;
+        if (!n) {
+            const DevNode d = rnodes[it.rnode];
+            if ((d.y & 3u) != 3u) {
+                nodes.push_back(DevNode{d.x, d.y & 3u});
+                st.push_back({nullptr, d.y >> 2, me, it.level + 1});
+                st.push_back({nullptr, it.rnode + 1, 0xFFFFFFFFu, it.level + 1});
+                continue;
+            }
+            LeafJob& j = jobs[job_of[it.rnode]];
+            if (!j.sub) {
+                nodes.push_back(DevNode{(uint32_t)refs.size(), (uint32_t)j.ids.size() << 2 | 3u});
+                refs.insert(refs.end(), j.ids.begin(), j.ids.end());
+                continue;
+            }
+            n = j.sub.get();
+        }
+        if (!n->leaf) {
+            uint32_t sp;
+            std::memcpy(&sp, &n->split, 4);
+            nodes.push_back(DevNode{sp, (uint32_t)n->axis});
+            st.push_back({n->right.get(), 0u, me, it.level + 1});
+            st.push_back({n->left.get(), 0u, 0xFFFFFFFFu, it.level + 1});
+        } else {
+            nodes.push_back(DevNode{(uint32_t)refs.size(), (uint32_t)n->ids.size() << 2 | 3u});
+            refs.insert(refs.end(), n->ids.begin(), n->ids.end());
+        }
+    }
 }
 
 uint32_t tree_height(const std::vector<uint64_t>& flat) {  // lib/kdtree.h:197-216

@@ -46,6 +46,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "tn_guide.h"
@@ -115,7 +116,9 @@ struct ChunkState {
 };
 
 struct RenderArgs {
-    SceneView sv;
+    SceneView sv;        // the tree the traversal kernels walk (refined tree of a TN_SCENE_REFINED scene)
+    SceneView sv_exact;  // the reference tree behind a refined one (rays the refined traversal marks)
+    bool refined;        // sv is a refined tree
     const float* normals;  // 9 per triangle
     const float4* albedo;
     const float4* emission;
@@ -353,17 +356,34 @@ struct DynamicChunks {
     }
 };
 
-template <bool EXH>
+// An unoccluded shadow ray: its contribution goes to the pixel (and, when
+// training, its radiance to the segment's record).
+__device__ __forceinline__ void shadow_lit(const RenderArgs& A, const Queues& Q, int sb, uint32_t k) {
+    accumulate(A.accum, __float_as_uint(Q.so[sb][k].w), Q.sc[sb][k]);
+    if (A.rec_x) {
+        const float4 x = Q.sx[sb][k];
+        float* dst = reinterpret_cast<float*>(A.rec_x + __float_as_uint(x.w));
+        atomicAdd(dst + 0, x.x);
+        atomicAdd(dst + 1, x.y);
+        atomicAdd(dst + 2, x.z);
+    }
+}
+
+// REF: the refined tree of a TN_SCENE_REFINED scene; rays tied at their
+// closest distance and rays with a zero direction component get the hit
+// record kFixId, which k_fix_hits replaces before anything reads it.
+template <bool EXH, bool REF>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
     RenderArgs A, Queues Q, int level, int buf) {
+    using WL = std::conditional_t<REF, CoopLdsTie, CoopLds>;
     __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ CoopLds wl[kRBlock / 64];
+    __shared__ WL wl[kRBlock / 64];
     const uint32_t n = min(Q.cnt[level], Q.cap);  // overflow is reported, never read
     const float4* qo = Q.o[buf];
     const float4* qd = Q.d[buf];
     uint4* hit = Q.hit;
     TraversalCounters cnt;
-    coop_loop<kRStack, EXH, false, false, false>(
+    coop_loop<kRStack, EXH, false, false, false, REF>(
         A.sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
         DynamicChunks{n, Q.xwork + 16 * kXcds * level},
         [&](uint32_t k, float3& o, float3& d, float&, uint32_t& scene) {
@@ -373,6 +393,10 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
             scene = scene_of_pixel(A, __float_as_uint(a.w));
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
+            if (REF && (T.tie || needs_reference(T.d))) {
+                hit[k] = make_uint4(kFixId, 0u, 0u, 0u);
+                return;
+            }
             const bool h = traversed && T.best_id != kInvalidId;
             hit[k] = make_uint4(h ? T.best_id : kInvalidId, __float_as_uint(h ? T.best_r : FLT_MAX),
                                 __float_as_uint(h ? T.best_s : 0.f), __float_as_uint(h ? T.best_t : 0.f));
@@ -381,7 +405,30 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
 }
 
+// The segments of one level that k_trace<..., true> marked, traced one per
+// lane on the reference tree (the reference's leaves and order decide them).
 template <bool EXH>
+__global__ __launch_bounds__(kRBlock) void k_fix_hits(RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ uint2 stk[kRStack * kRBlock];
+    const uint32_t n = min(Q.cnt[level], Q.cap);
+    TraversalCounters cnt;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (Q.hit[i].x != kFixId) continue;
+        const float4 a = Q.o[buf][i], b = Q.d[buf][i];
+        float r = FLT_MAX, s = 0.f, t = 0.f;
+        const uint32_t id = traverse<kRStack, EXH, false, false, false>(
+            A.sv_exact, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), 0.f, r, s, t, stk + threadIdx.x,
+            kRBlock, cnt);
+        const bool h = id != kInvalidId;
+        Q.hit[i] = make_uint4(h ? id : kInvalidId, __float_as_uint(h ? r : FLT_MAX), __float_as_uint(h ? s : 0.f),
+                              __float_as_uint(h ? t : 0.f));
+    }
+    if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
+}
+
+// Shadow rays (any hit up to tmax).  REF: the refined tree; a ray with a zero
+// direction component is marked (sign bit of its tmax) for k_fix_shadows.
+template <bool EXH, bool REF>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
     RenderArgs A, Queues Q, int level) {
     __shared__ uint2 stk[kRStack * kRBlock];
@@ -400,18 +447,36 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
             scene = scene_of_pixel(A, __float_as_uint(a.w));
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
-            if (!traversed || T.best_id == kInvalidId) {
-                accumulate(A.accum, __float_as_uint(Q.so[sb][k].w), Q.sc[sb][k]);
-                if (A.rec_x) {
-                    const float4 x = Q.sx[sb][k];
-                    float* dst = reinterpret_cast<float*>(A.rec_x + __float_as_uint(x.w));
-                    atomicAdd(dst + 0, x.x);
-                    atomicAdd(dst + 1, x.y);
-                    atomicAdd(dst + 2, x.z);
-                }
+            if (REF && needs_reference(T.d)) {
+                float* w = reinterpret_cast<float*>(Q.sd[sb] + k) + 3;
+                *w = __uint_as_float(__float_as_uint(*w) | 0x80000000u);
+                return;
             }
+            if (!traversed || T.best_id == kInvalidId) shadow_lit(A, Q, sb, k);
         },
         cnt);
+    if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
+}
+
+// The shadow rays of one level that k_shadow<..., true> marked, tested one
+// per lane on the reference tree.
+template <bool EXH>
+__global__ __launch_bounds__(kRBlock) void k_fix_shadows(RenderArgs A, Queues Q, int level) {
+    __shared__ uint2 stk[kRStack * kRBlock];
+    const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
+    const int sb = level & 1;
+    TraversalCounters cnt;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 b = Q.sd[sb][i];
+        const uint32_t wbits = __float_as_uint(b.w);
+        if (!(wbits >> 31)) continue;
+        const float4 a = Q.so[sb][i];
+        float r = FLT_MAX, s = 0.f, t = 0.f;
+        const uint32_t id = traverse<kRStack, EXH, false, true, false>(
+            A.sv_exact, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), __uint_as_float(wbits & 0x7FFFFFFFu),
+            r, s, t, stk + threadIdx.x, kRBlock, cnt);
+        if (id == kInvalidId) shadow_lit(A, Q, sb, i);
+    }
     if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
 }
 
@@ -593,8 +658,11 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
     const int ob = buf ^ 1;
     const int sb = level & 1;
     TraversalCounters cnt;
+    // the raytracer's shadow test darkens the mirror child too: it runs on the
+    // reference tree itself (no marks to resolve afterwards)
+    const SceneView& sv = A.refined ? A.sv_exact : A.sv;
     coop_loop<kRStack, EXH, false, true, false>(
-        A.sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
+        sv, stk + threadIdx.x, kRBlock, wl + threadIdx.x / 64,
         DynamicChunks{n, Q.xwork + 16 * kXcds * (kMaxLevels + level)},
         [&](uint32_t k, float3& o, float3& d, float& tmax, uint32_t& scene) {
             const float4 a = Q.so[sb][k], b = Q.sd[sb][k];
@@ -1087,6 +1155,12 @@ struct RenderContext {
     hipEvent_t ev_fork = nullptr, ev_join[kPipes] = {};
     float* jitter = nullptr;  // reference pixel jitter (TN_RENDER_REF_JITTER)
     size_t jitter_cap = 0;
+    // graph replay (TN_RENDER_GRAPH): the capture stream and the executable
+    // graph of the last captured chunk sequence, updated in place while the
+    // topology stays the same
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t graph_instantiations = 0, graph_updates = 0;
 };
 
 RenderContext& render_context(int device) {
@@ -1110,6 +1184,8 @@ __global__ void k_chunk_setup(ChunkState* cs, unsigned long long total, uint32_t
 // forest of a batch.
 struct RenderGeom {
     SceneView sv;
+    SceneView exact;       // TN_SCENE_REFINED: the reference tree behind the refined sv
+    bool refined = false;
     const float* normals;
     const float4* albedo;
     const float4* emission;
@@ -1125,7 +1201,9 @@ struct RenderGeom {
 
 RenderGeom geom_of(const tn_scene* sc) {
     RenderGeom g;
-    g.sv = sc->view();
+    g.sv = sc->refined ? sc->fview() : sc->view();
+    g.exact = sc->view();
+    g.refined = sc->refined;
     g.normals = sc->d_normals;
     g.albedo = sc->d_albedo;
     g.emission = sc->d_emission;
@@ -1299,6 +1377,8 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
 
     RenderArgs A0{};
     A0.sv = G.sv;
+    A0.sv_exact = G.exact;
+    A0.refined = G.refined;
     A0.normals = G.normals;
     A0.albedo = G.albedo;
     A0.emission = G.emission;
@@ -1382,9 +1462,18 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         else k_raygen<<<rgrid, kRBlock, 0, s>>>(Ap, Qp);
         const bool shadows = renderer != TN_RENDERER_RAYCASTER;
         int buf = 0;
+        const bool refd = G.refined;
+        const dim3 fgrid((uint32_t)G.cus * 4u);
         for (int lv = 0; lv < levels; ++lv) {
-            if (exh) k_trace<true><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
-            else k_trace<false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+            if (refd) {
+                if (exh) k_trace<true, true><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+                else k_trace<false, true><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+                if (exh) k_fix_hits<true><<<fgrid, kRBlock, 0, s>>>(Ap, Qp, lv, buf);
+                else k_fix_hits<false><<<fgrid, kRBlock, 0, s>>>(Ap, Qp, lv, buf);
+            } else {
+                if (exh) k_trace<true, false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+                else k_trace<false, false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+            }
             if (renderer == TN_RENDERER_RAYCASTER) {
                 k_cast<<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
                 buf ^= 1;
@@ -1400,9 +1489,14 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
             if (renderer == TN_RENDERER_RAYTRACER) {
                 if (exh) k_rt_shadow<true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv, buf);
                 else k_rt_shadow<false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv, buf);
+            } else if (refd) {
+                if (exh) k_shadow<true, true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
+                else k_shadow<false, true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
+                if (exh) k_fix_shadows<true><<<fgrid, kRBlock, 0, side>>>(Ap, Qp, lv);
+                else k_fix_shadows<false><<<fgrid, kRBlock, 0, side>>>(Ap, Qp, lv);
             } else {
-                if (exh) k_shadow<true><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
-                else k_shadow<false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
+                if (exh) k_shadow<true, false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
+                else k_shadow<false, false><<<tgrid, tblock, 0, side>>>(Ap, Qp, lv);
             }
             chk(hipEventRecord(rb.ev_shadow[lv], side));
             buf ^= 1;
@@ -1438,22 +1532,38 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     const uint64_t rounds = (nchunks + npipes - 1) / npipes;
     auto t0 = std::chrono::steady_clock::now();
     if (shape.graph && rounds > 1) {
-        hipStream_t cap_stream = nullptr;
+        // The chunk sequence is captured on every call (host-side recording;
+        // its kernel arguments change from call to call) and applied to the
+        // executable graph kept from the last call by hipGraphExecUpdate,
+        // which needs the same topology; only a changed topology (levels,
+        // renderer, guided or training work) instantiates a new one.
+        if (!C.cap_stream) TN_RHIP(hipStreamCreateWithFlags(&C.cap_stream, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        TN_RHIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
-        hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
+        hipError_t e = hipStreamBeginCapture(C.cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            const hipError_t es = round(cap_stream);  // the other streams join and leave the capture
-            e = hipStreamEndCapture(cap_stream, &graph);
+            const hipError_t es = round(C.cap_stream);  // the other streams join and leave the capture
+            e = hipStreamEndCapture(C.cap_stream, &graph);
             if (e == hipSuccess) e = es;
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        for (uint64_t c = 0; e == hipSuccess && c < rounds; ++c) e = hipGraphLaunch(exec, stream);
+        if (e == hipSuccess && C.exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult why = hipGraphExecUpdateSuccess;
+            if (hipGraphExecUpdate(C.exec, graph, &bad, &why) != hipSuccess || why != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(C.exec);
+                C.exec = nullptr;
+            } else {
+                ++C.graph_updates;
+            }
+        }
+        if (e == hipSuccess && !C.exec) {
+            e = hipGraphInstantiate(&C.exec, graph, nullptr, nullptr, 0);
+            if (e == hipSuccess) ++C.graph_instantiations;
+            else C.exec = nullptr;
+        }
+        for (uint64_t c = 0; e == hipSuccess && c < rounds; ++c) e = hipGraphLaunch(C.exec, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        (void)hipStreamDestroy(cap_stream);
         if (e != hipSuccess) return fail(hip_error(e, "captured chunk sequence"));
     } else {
         for (uint64_t c = 0; c < rounds; ++c) {

@@ -39,6 +39,13 @@ bool flat_to_device(const std::vector<uint64_t>& flat, uint32_t num_triangles,
 
 uint32_t tree_height(const std::vector<uint64_t>& flat);
 
+// The reference device tree (rnodes / rrefs over box6) with every leaf's
+// references filtered to triangles near the leaf and larger leaves replaced
+// by rendering-variant sub-trees; inner reference nodes and their order stay.
+void refine_reference_tree(const float* verts9, const std::vector<DevNode>& rnodes,
+                           const std::vector<uint32_t>& rrefs, const float box6[6], bool parallel,
+                           std::vector<DevNode>& nodes, std::vector<uint32_t>& refs, uint32_t& height);
+
 // SAH build that keeps empty children as empty leaves, straight into the
 // device layout (no reference FlatNode array).
 void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vector<DevNode>& nodes,

@@ -28,6 +28,15 @@ struct tn_scene {
     float4* d_tris = nullptr;
     float4* d_lrec = nullptr;    // records per reference: 2 x nrefs first halves, then second halves
     uint32_t nrefs = 0;
+    // TN_SCENE_REFINED: the refined reference tree (same layout, own records per reference)
+    bool refined = false;
+    bool parallel_build = true;
+    std::vector<tn::DevNode> fnodes;
+    std::vector<uint32_t> frefs;
+    uint32_t fheight = 0;
+    uint2* d_fnodes = nullptr;
+    uint32_t* d_frefs = nullptr;
+    float4* d_flrec = nullptr;
     float* d_normals = nullptr;  // 9 per triangle
     float4* d_albedo = nullptr;
     float4* d_emission = nullptr;
@@ -58,6 +67,16 @@ struct tn_scene {
 
     // Device memory of at least `bytes` in d_stage (caller holds stage_mu).
     int reserve_stage(size_t bytes);
+
+    // the refined tree (TN_SCENE_REFINED scenes only)
+    tn::SceneView fview() const {
+        tn::SceneView v = view();
+        v.nodes = d_fnodes;
+        v.refs = d_frefs;
+        v.lra = d_flrec;
+        v.lrb = d_flrec ? d_flrec + 2 * frefs.size() : nullptr;
+        return v;
+    }
 
     tn::SceneView view() const {
         tn::SceneView v;

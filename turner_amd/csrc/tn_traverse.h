@@ -39,6 +39,9 @@
 namespace tn {
 
 constexpr uint32_t kInvalidId = 1u << 30;
+// Hit record of a ray a traversal of a refined tree leaves to the reference
+// tree (not a triangle id: ids are below 2^30).
+constexpr uint32_t kFixId = 0xFFFFFFFFu;
 constexpr float kDirEps = 0.00001f;  // lib/types.h:14
 constexpr float kExitMargin = 1.0f + 1.0f / 65536.0f;
 constexpr int kMaxRestarts = 4096;
@@ -84,6 +87,7 @@ struct Trav {
     int head, count, restarts;
     uint32_t visits;  // leaves visited
     bool lost;
+    bool tie;  // another triangle was hit at exactly best_r (tie detection only)
     uint32_t best_id;
     float best_r, best_s, best_t;
 };
@@ -104,9 +108,10 @@ __device__ __forceinline__ bool certainly_negative(float num, float den) {
 // A candidate is also rejected when r is not below `bound` (closest hit: the
 // current best, which it must beat strictly) or, with ANY, above it.
 // Returns 2 on a hit, 1 for a miss decided after reading the whole record,
-// 0 for a miss decided from its first 32 bytes.  pa: the record's first half
-// (v0, uv) (n, vv); pc: its second half (u, uu) (v, denom).
-template <bool ANY>
+// 0 for a miss decided from its first 32 bytes; with TIE (closest hit) 3 for
+// a hit at exactly r == bound, which does not replace the best.  pa: the
+// record's first half (v0, uv) (n, vv); pc: its second half (u, uu) (v, denom).
+template <bool ANY, bool TIE = false>
 __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ pa, const float4* __restrict__ pc,
                                              float3 o, float3 d, float bound, float& r, float& s, float& t) {
     const float4 a = pa[0];  // v0, uv
@@ -124,7 +129,7 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ pa, cons
     }
     const float rr = nom / den;
     if (rr < 0) return 0;
-    if (ANY ? !(rr <= bound) : !(rr < bound)) return 0;
+    if ((ANY || TIE) ? !(rr <= bound) : !(rr < bound)) return 0;
     const float4 c = pc[0];  // u, uu
     const float4 e = pc[1];  // v, denom
     const float px = o.x + rr * d.x, py = o.y + rr * d.y, pz = o.z + rr * d.z;
@@ -151,7 +156,7 @@ __device__ __forceinline__ int leaf_test_ref(const float4* __restrict__ pa, cons
     r = rr;
     s = ss;
     t = tt;
-    return 2;
+    return (TIE && !ANY && rr == bound) ? 3 : 2;
 }
 
 // Moeller & Trumbore 1997 in FP32 on the record's v0, e1 = u, e2 = v.
@@ -180,6 +185,12 @@ __device__ __forceinline__ int leaf_test_mt(const float4* __restrict__ pa, const
     t = v;
     return 2;
 }
+
+// A direction with a zero component is traversed as if that component were
+// 1e-5 (kdtree.cpp:503-511): the traversal path then bends away from the ray
+// itself, and only the reference tree's own leaves reproduce the reference's
+// result for it.
+__device__ __forceinline__ bool needs_reference(float3 d) { return d.x == 0.f || d.y == 0.f || d.z == 0.f; }
 
 // Fixes the direction and clips the ray to the root box (kdtree.cpp:519-535)
 // of its scene (the tree of `scene` in a forest).  Returns false when the root
@@ -228,6 +239,7 @@ __device__ __forceinline__ bool trav_init(Trav& T, const SceneView& sc, float3 o
     T.restarts = 0;
     T.visits = 0;
     T.lost = false;
+    T.tie = false;
     T.best_id = kInvalidId;
     T.best_r = FLT_MAX;
     T.best_s = 0.f;
@@ -563,6 +575,14 @@ struct CoopLds {
     uint32_t mark[64];
 };
 
+// With tie detection: per lane also the largest key (~r bits, pair index) of
+// the candidates, i.e. the last pair among those at the smallest distance,
+// and so a second hit at exactly the winner's distance when it differs from
+// the winner.
+struct CoopLdsTie : CoopLds {
+    unsigned long long slot2[64];
+};
+
 
 // Wave-cooperative traversal loop.  Every round, each lane with a ray
 // descends to its next leaf; then the wave pools the (ray, triangle
@@ -582,10 +602,16 @@ struct CoopLds {
 //                      work is left (may update begin/end);
 //   load(k, o, d, tmax, scene), store(k, T, traversed): per-ray I/O (scene:
 //                      the ray's tree in a forest, 0 otherwise).
-template <int S, bool EXH, bool MT, bool ANY, bool COUNT, class Chunk, class Load, class Store>
+// TIE (closest hit, reference leaf test, WL = CoopLdsTie): T.tie tells at the
+// end whether a second triangle was hit at exactly the final best distance
+// (in the same leaf, or in a later leaf at the earlier best); the winner among
+// such ties depends on the tree's visit order.
+template <int S, bool EXH, bool MT, bool ANY, bool COUNT, bool TIE = false, class WL = CoopLds, class Chunk,
+          class Load, class Store>
 __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int stride,
-                                          CoopLds* wl, Chunk chunk, Load load,
+                                          WL* wl, Chunk chunk, Load load,
                                           Store store, TraversalCounters& cnt) {
+    static_assert(!TIE || (!ANY && !MT), "tie detection is for the reference leaf test's closest hit");
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t next = 0, end = 0, ray = kNone;
@@ -625,6 +651,7 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
         const float bound_own = ANY ? tmx : T.best_r;
         wl->slot[lane] = ~0ull;
         wl->mark[lane] = 0u;
+        if constexpr (TIE) wl->slot2[lane] = 0ull;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // owner of pair base + lane; called with all lanes active
         uint32_t stamp = 0;
@@ -644,25 +671,34 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
             const float3 rd = make_float3(__shfl(T.d.x, own, 64), __shfl(T.d.y, own, 64),
                                           __shfl(T.d.z, own, 64));
             const float bnd = __shfl(bound_own, own, 64);
+            uint32_t best_own = 0;
+            if constexpr (TIE) best_own = (uint32_t)__shfl((int)T.best_id, own, 64);
             if (item < total) {
                 const size_t r = rb + item;  // reference slot of the pair
                 float rr, ss, tt;
-                const int code = MT ? leaf_test_mt<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt)
-                                    : leaf_test_ref<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt);
+                int code = MT ? leaf_test_mt<ANY>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt)
+                              : leaf_test_ref<ANY, TIE>(sc.lra + 2 * r, sc.lrb + 2 * r, ro, rd, bnd, rr, ss, tt);
                 if (COUNT) {
                     cnt.refs++;
                     cnt.tests++;
                     cnt.full += code >= 1 ? 1u : 0u;
                 }
-                if (code == 2) {
+                // a hit at exactly the current best distance ties only when it
+                // is another triangle (a triangle spanning several leaves is
+                // met again in later ones)
+                if (TIE && code == 3 && sc.refs[r] == best_own) code = 1;
+                if (code >= 2) {
                     // r >= 0 here; -0 maps to +0 so the key orders like the reference's <
                     const uint32_t rbits = __float_as_uint(rr + 0.0f);
-                    atomicMin(&wl->slot[own], ((unsigned long long)rbits << 32) | item);
+                    if (code == 2) atomicMin(&wl->slot[own], ((unsigned long long)rbits << 32) | item);
+                    if constexpr (TIE) atomicMax(&wl->slot2[own], ((unsigned long long)(~rbits) << 32) | item);
                 }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const unsigned long long key = wl->slot[lane];
+        unsigned long long key2 = 0ull;
+        if constexpr (TIE) key2 = wl->slot2[lane];
         // every active lane moves on after its leaf, empty leaves included
         if (ray != kNone) {
             if (lcnt && key != ~0ull) {
@@ -675,10 +711,16 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                 T.best_s = ss;
                 T.best_t = tt;
                 T.best_id = id;
+                // a new best: tied iff the last pair at its distance is another pair
+                if constexpr (TIE)
+                    T.tie = key2 != 0ull && (uint32_t)~(uint32_t)(key2 >> 32) == (uint32_t)(key >> 32) &&
+                            (uint32_t)key2 != (uint32_t)key;
                 if (ANY) {
                     store(ray, T, true);
                     ray = kNone;
                 }
+            } else if (TIE && lcnt && key2 != 0ull) {
+                T.tie = true;  // a hit at exactly the current best distance
             }
             if (ray != kNone) {
                 bool done = trav_after_leaf<S, EXH, ANY>(T, tmx, stk, stride, cnt);
