@@ -51,6 +51,11 @@ def host_cpu():
 METRIC = "Mrays/s + time-to-relMSE 0.01, guided vs plain tracer, 1 MI355X"
 
 
+def note(msg):
+    """a progress line on stderr (stdout carries only the JSON line)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +195,8 @@ def main():
     assert np.all(hid != T.TN_INVALID_ID), "a ray escaped the closed scene"
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
     ref_achieved = ref_bytes / (ref_ms * 1e-3) / 1e9
+    note(f"intersect: refined {value:.1f} Mrays/s, reference tree {n / (ref_ms * 1e-3) / 1e6:.1f} Mrays/s, "
+         f"bit-exact {exact}")
     traffic = traffic_of("traffic_intersect_refined_level8_2p24.txt")
     ref_traffic = traffic_of("traffic_intersect_level8_2p24.txt")
     info = tree.info
@@ -271,12 +278,15 @@ def main():
                        f"{ncores} threads, {csec:.1f} s; GPU ids identical on the sample"),
             "host": cpu,
         }
+        note(f"intersect CPU baseline: {out['cpu_baseline']['value']} Mrays/s on {ncores} threads")
     if not args.no_render:
         out["render"] = render_throughput(args)
     if not args.no_batch:
         out["batch"] = batch_throughput(args)
+        note("batch done")
     if not args.no_guided:
         out["guided"] = guided_vs_plain(args)
+        note("guided done")
     print(json.dumps(out), flush=True)
 
 
@@ -558,6 +568,7 @@ def render_throughput(args):
             "host_build_s": round(build_s, 2), "device_nodes": tree.num_nodes(),
             "hits": hits[label],
         }
+        note(f"render {label}: {out[label]['closest_hit_mrays_s']} M closest-hit rays/s")
         if el:
             keep = tree
         elif label == "reference_tree":

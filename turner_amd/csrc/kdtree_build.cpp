@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 #include <limits>
@@ -228,6 +229,10 @@ constexpr int kRefTrav = 15;    // lib/kdtree.cpp:178-179
 // on the reference's CPU; 60 gave the fastest renders of 15, 30, 60, 120 and
 // 240 on the 1.3 M-triangle scenes (0.83 M nodes instead of 2.7 M).
 constexpr int kRenderTrav = 60;
+// The same for the sub-trees of a refined reference tree (whose leaf tests
+// also track exact ties): 120 gave the fastest renders of 30, 60, 120 and
+// 240 on the 1.3 M-triangle scene.
+constexpr int kRefinedTrav = 120;
 inline float cost_(int trav, float lr, float rr, size_t nl, size_t nr) {
     return lambda_(nl, nr) * (trav + kIsect * (lr * nl + rr * nr));
 }
@@ -239,8 +244,9 @@ class Builder {
 public:
     // keep_tol: how far beyond a node's box a triangle's bounding box may end
     // and the triangle still be kept when clipping removes it (rendering variant)
-    Builder(const float* verts, bool parallel, bool keep_empty = false, float keep_tol = kEps)
-        : v_(verts), parallel_(parallel), keep_empty_(keep_empty), keep_tol_(keep_tol) {
+    Builder(const float* verts, bool parallel, bool keep_empty = false, float keep_tol = kEps,
+            int trav = kRenderTrav)
+        : v_(verts), parallel_(parallel), keep_empty_(keep_empty), trav_(trav), keep_tol_(keep_tol) {
         unsigned hc = std::thread::hardware_concurrency();
         budget_.store(parallel ? (int)(hc ? hc : 4) - 1 : 0);
     }
@@ -248,8 +254,9 @@ public:
     std::unique_ptr<Node> build(std::vector<uint32_t> tris, const Box& box, int depth = 0) {
         if (tris.empty()) return keep_empty_ ? leaf({}) : nullptr;
         if (tris.size() <= 3 || area(box) == 0) return leaf(std::move(tris));
-        // the rendering variant's conservative classification can give both
-        // children every triangle: such a split separates nothing
+        // in the rendering variant, triangles kept by their clamped bounding
+        // box can straddle every split: a depth cap, and a split that gives
+        // both children every triangle separates nothing
         if (keep_empty_ && depth >= kMaxRenderDepth) return leaf(std::move(tris));
         std::vector<uint32_t> L, R;
         int ax;
@@ -305,10 +312,6 @@ private:
                      float& out_pos, std::vector<uint32_t>& L, std::vector<uint32_t>& R) const {
         std::vector<Event> ev[3];
         for (auto& e : ev) e.reserve(2 * tris.size());
-        // rendering variant: the clipped box of every triangle taking part, for
-        // the conservative pass after the classification
-        std::vector<std::pair<uint32_t, Box>> kept;
-        if (keep_empty_) kept.reserve(tris.size());
         size_t num = 0;
         // Rendering variant: in a node thinner than 4 kEps along some axis the
         // thick-plane clip is unreliable (the tolerance zones of its two planes
@@ -345,7 +348,6 @@ private:
             // on the node's face) hits it inside this node
             if (!keep_empty_ && empty(cb)) continue;
             ++num;
-            if (keep_empty_) kept.emplace_back(id, cb);
             for (int a = 0; a < 3; ++a) {
                 if (planar(cb, a)) {
                     ev[a].push_back(Event{id, at(cb.lo, a), at(cb.lo, a), kPlanar});
@@ -381,45 +383,19 @@ private:
                 while (i < n && E[i].point == p && E[i].type == kStarting) { ++ps; ++i; }
                 np = pp;
                 nr -= pp + pe;
-                if (keep_empty_) {
-                    // Rendering variant: the split goes midway between this
-                    // event position and the next, so no vertex lies on a split
-                    // plane (a ray aimed at such a vertex could reach it through
-                    // the child the triangle does not touch); everything that
-                    // started at or before p is left of it, everything not yet
-                    // ended right of it.  No split within 1/1024 of the node's
-                    // extent from its faces: such a child cuts next to nothing
-                    // off, yet the empty-side bonus would keep choosing it.
-                    if (i < n) {
-                        const float m = 0.5f * (p + E[i].point);
-                        const float margin = (at(box.hi, a) - at(box.lo, a)) * (1.0f / 1024.0f);
-                        if (p < m && m < E[i].point && m > at(box.lo, a) + margin && m < at(box.hi, a) - margin) {
-                            Box lb, rb;
-                            split(box, a, m, lb, rb);
-                            const float c = cost_(kRenderTrav, area(lb) / A, area(rb) / A, nl + pp + ps, nr);
-                            if (c < min_cost) {
-                                min_cost = c;
-                                min_ax = a;
-                                min_pos = m;
-                                min_left = true;
-                            }
-                        }
-                    }
-                } else {
-                    Box lb, rb;  // lib/kdtree.cpp:218-238
-                    split(box, a, p, lb, rb);
-                    float lr = area(lb) / A, rr = area(rb) / A;
-                    const int trav = keep_empty_ ? kRenderTrav : kRefTrav;
-                    float lc = cost_(trav, lr, rr, nl + np, nr);
-                    float rc = cost_(trav, lr, rr, nl, np + nr);
-                    float c = lc < rc ? lc : rc;
-                    bool left = lc < rc;
-                    if (c < min_cost) {
-                        min_cost = c;
-                        min_ax = a;
-                        min_pos = p;
-                        min_left = left;
-                    }
+                Box lb, rb;  // lib/kdtree.cpp:218-238
+                split(box, a, p, lb, rb);
+                float lr = area(lb) / A, rr = area(rb) / A;
+                const int trav = keep_empty_ ? trav_ : kRefTrav;
+                float lc = cost_(trav, lr, rr, nl + np, nr);
+                float rc = cost_(trav, lr, rr, nl, np + nr);
+                float c = lc < rc ? lc : rc;
+                bool left = lc < rc;
+                if (c < min_cost) {
+                    min_cost = c;
+                    min_ax = a;
+                    min_pos = p;
+                    min_left = left;
                 }
                 nl += ps + pp;
                 np = 0;
@@ -442,24 +418,6 @@ private:
                 R.push_back(e.id);
             }
         }
-        if (keep_empty_) {
-            // Conservative assignment: every triangle whose clipped box
-            // reaches the split plane (within a few ulps of its position)
-            // goes into both children.  The events alone miss a clipped box
-            // thinner than kEps, which counts as planar and goes by its low
-            // coordinate to one side even when it straddles the split (deep in
-            // a rendering tree nodes are thinner than kEps), and a vertex that
-            // rounding leaves exactly on the plane.
-            const float tol = std::ldexp(std::abs(min_pos), -21);
-            std::vector<uint32_t> sl(L), sr(R);
-            std::sort(sl.begin(), sl.end());
-            std::sort(sr.begin(), sr.end());
-            for (const auto& [id, cb] : kept) {
-                if (at(cb.lo, min_ax) > min_pos + tol || at(cb.hi, min_ax) < min_pos - tol) continue;
-                if (!std::binary_search(sl.begin(), sl.end(), id)) L.push_back(id);
-                if (!std::binary_search(sr.begin(), sr.end(), id)) R.push_back(id);
-            }
-        }
         out_ax = min_ax;
         out_pos = min_pos;
         return min_cost;
@@ -468,6 +426,7 @@ private:
     const float* v_;
     bool parallel_;
     bool keep_empty_;  // empty children become empty leaves instead of being dropped
+    int trav_;         // traversal cost of the rendering variant
     float keep_tol_;
     std::atomic<int> budget_{0};
 };
@@ -529,6 +488,40 @@ void precompute_triangles(const float* verts9, uint32_t n, float* out16) {
         o[9] = v.x; o[10] = v.y; o[11] = v.z;
         o[12] = uv; o[13] = vv; o[14] = uu; o[15] = uv * uv - uu * vv;
     }
+}
+
+// Every leaf below n gets every candidate triangle that meets the leaf's cell
+// enlarged by tol (separating-axis test), besides the builder's own choice,
+// whose thick-plane clipping can drop a sliver beyond a split plane, a
+// triangle whose clipped box is thinner than the clipping tolerance, or one
+// whose vertex lies on a split plane: a hit point inside a cell is then found
+// in that cell.  Leaf order: ascending triangle id.  Subtrees with many
+// candidates run on threads of their own when parallel.
+void cover_tree(const float* verts9, Node* n, const Box& box, const std::vector<uint32_t>& cand, float tol,
+                bool parallel) {
+    std::vector<uint32_t> mine;
+    mine.reserve(cand.size());
+    for (uint32_t id : cand) {
+        const float* t = verts9 + 9 * (size_t)id;
+        if (near_box(tri_box(t), box, tol) && tri_meets_box(t, box, tol)) mine.push_back(id);
+    }
+    if (!n->leaf) {
+        Box l, r;
+        split(box, n->axis, n->split, l, r);
+        if (parallel && mine.size() > 65536) {
+            auto fut = std::async(std::launch::async,
+                                  [&]() { cover_tree(verts9, n->left.get(), l, mine, tol, true); });
+            cover_tree(verts9, n->right.get(), r, mine, tol, true);
+            fut.get();
+        } else {
+            cover_tree(verts9, n->left.get(), l, mine, tol, parallel);
+            cover_tree(verts9, n->right.get(), r, mine, tol, parallel);
+        }
+        return;
+    }
+    n->ids.insert(n->ids.end(), mine.begin(), mine.end());
+    std::sort(n->ids.begin(), n->ids.end());
+    n->ids.erase(std::unique(n->ids.begin(), n->ids.end()), n->ids.end());
 }
 
 void build_kdtree(const float* verts9, uint32_t n, bool parallel, std::vector<uint64_t>& flat,
@@ -628,8 +621,11 @@ void build_device_tree(const float* verts9, uint32_t n, bool parallel, std::vect
 // magnitude, far above the rounding of any hit point or split crossing: the
 // others are triangles the reference's unbounded-box quirk sank into the
 // leaf from an ancestor), and a leaf keeping more than four is replaced by a
-// rendering-variant sub-tree over its box (SAH node step 60, clipped-away
-// triangles kept within tol).  Closest hits then equal the reference tree's
+// rendering-variant sub-tree over its box (SAH node step kRefinedTrav,
+// clipped-away triangles kept within tol) whose every leaf then also gets
+// each of those triangles that meets its cell (cover_tree, tolerance 2^-26 of
+// the magnitude, above the rounding of the FP64 overlap test): a hit point
+// inside a cell is found in that cell.  Closest hits then equal the reference tree's
 // except where two triangles are hit at exactly the same distance (the
 // winner depends on the order inside a leaf) or a direction component is
 // zero (its 1e-5 substitute bends the traversal path away from the ray):
@@ -674,28 +670,6 @@ void refine_reference_tree(const float* verts9, const std::vector<DevNode>& rnod
             st.push_back({it.node + 1, l});
         }
     }
-    // Every leaf of a sub-tree gets every triangle of the reference leaf that
-    // meets the leaf's cell enlarged by tol, besides the builder's own choice
-    // (whose thick-plane clipping can drop a sliver beyond a split plane):
-    // a hit point inside a cell is then found in that cell.  Leaf order:
-    // ascending triangle id (exact ties are settled on the reference tree).
-    std::function<void(Node*, const Box&, const std::vector<uint32_t>&, float)> cover =
-        [&](Node* n, const Box& box, const std::vector<uint32_t>& cand, float tol) {
-            std::vector<uint32_t> mine;
-            mine.reserve(cand.size());
-            for (uint32_t id : cand)
-                if (tri_meets_box(verts9 + 9 * (size_t)id, box, tol)) mine.push_back(id);
-            if (!n->leaf) {
-                Box l, r;
-                split(box, n->axis, n->split, l, r);
-                cover(n->left.get(), l, mine, tol);
-                cover(n->right.get(), r, mine, tol);
-                return;
-            }
-            n->ids.insert(n->ids.end(), mine.begin(), mine.end());
-            std::sort(n->ids.begin(), n->ids.end());
-            n->ids.erase(std::unique(n->ids.begin(), n->ids.end()), n->ids.end());
-        };
     auto refine = [&](LeafJob& j) {
         float mag = 0.f;
         for (int ax = 0; ax < 3; ++ax)
@@ -706,9 +680,9 @@ void refine_reference_tree(const float* verts9, const std::vector<DevNode>& rnod
         for (uint32_t id : j.ids)
             if (near_box(tri_box(verts9 + 9 * (size_t)id), j.box, tol)) keep.push_back(id);
         if (keep.size() > 4) {
-            Builder B(verts9, false, true, tol);
+            Builder B(verts9, false, true, tol, kRefinedTrav);
             j.sub = B.build(keep, j.box);
-            cover(j.sub.get(), j.box, keep, tol);
+            cover_tree(verts9, j.sub.get(), j.box, keep, std::ldexp(mag, -26), false);
             j.ids.clear();
         } else {
             j.ids = std::move(keep);

@@ -61,6 +61,8 @@ constexpr int kRBlock = 256;
 // measured no faster on small or large launches than 8)
 constexpr int kRStack = kStack;
 constexpr uint32_t kMaxLevels = 64;
+constexpr uint32_t kFixCap = 1u << 16;
+constexpr uint32_t kFixHits = 4 * kMaxLevels + 2, kFixShadows = 5 * kMaxLevels + 2;
 constexpr uint64_t kSceneSeedStep = 0x9E3779B97F4A7C15ull;
 // scenes with at most this many emitter triangles pick one by solid angle
 constexpr uint32_t kSolidAngleSelect = 4;
@@ -84,8 +86,13 @@ struct Queues {
     uint32_t scap;  // capacity of the shadow queue
     // device counters: [0..kMaxLevels) path counts, [kMaxLevels..2k) shadow
     // counts, [2k..3k) trace work, [3k..4k) shadow work, [4k] overflow,
-    // [4k + 1] raycaster hits
+    // [4k + 1] raycaster hits, [4k + 2..5k + 2) segments and [5k + 2..6k + 2)
+    // shadow rays a refined traversal handed to the reference tree
     uint32_t* cnt;
+    // lists of those rays: [0] the segments of the level being traced,
+    // [1 + (level & 1)] the shadow rays of a level; kFixCap entries each (a
+    // list that overflows is replaced by a scan of the whole queue)
+    uint32_t* fix;
     // chunk counters of the trace / shadow kernels: [kind][level][XCD], 16
     // words apart (one per 64-byte line)
     uint32_t* xwork;
@@ -235,6 +242,7 @@ __global__ void k_chunk_begin(uint32_t* cnt, uint32_t* xwork, ChunkState* cs) {
     const uint32_t count = (uint32_t)(left < cs->chunk ? left : cs->chunk);
     // the overflow word [4 kMaxLevels] lives for the whole render
     if (i < 4 * kMaxLevels || i == 4 * kMaxLevels + 1) cnt[i] = (i == 0) ? count : 0u;
+    for (uint32_t k = kFixHits + i; k < 6 * kMaxLevels + 2; k += blockDim.x) cnt[k] = 0u;
     for (uint32_t k = i; k < 2 * kMaxLevels * kXcds; k += blockDim.x) xwork[16 * k] = 0u;
     __syncthreads();
     if (i == 0) {
@@ -371,7 +379,8 @@ __device__ __forceinline__ void shadow_lit(const RenderArgs& A, const Queues& Q,
 
 // REF: the refined tree of a TN_SCENE_REFINED scene; rays tied at their
 // closest distance and rays with a zero direction component get the hit
-// record kFixId, which k_fix_hits replaces before anything reads it.
+// record kFixId, which k_fix_hits replaces before anything reads it, and go
+// on the level's fix list.
 template <bool EXH, bool REF>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
     RenderArgs A, Queues Q, int level, int buf) {
@@ -395,6 +404,8 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
         [&](uint32_t k, const Trav& T, bool traversed) {
             if (REF && (T.tie || needs_reference(T.d))) {
                 hit[k] = make_uint4(kFixId, 0u, 0u, 0u);
+                const uint32_t slot = atomicAdd(Q.cnt + kFixHits + level, 1u);
+                if (slot < kFixCap) Q.fix[slot] = k;
                 return;
             }
             const bool h = traversed && T.best_id != kInvalidId;
@@ -406,14 +417,20 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
 }
 
 // The segments of one level that k_trace<..., true> marked, traced one per
-// lane on the reference tree (the reference's leaves and order decide them).
+// lane on the reference tree (the reference's leaves and order decide them):
+// the entries of the fix list, or every marked entry of the queue when the
+// list overflowed.
 template <bool EXH>
 __global__ __launch_bounds__(kRBlock) void k_fix_hits(RenderArgs A, Queues Q, int level, int buf) {
     __shared__ uint2 stk[kRStack * kRBlock];
-    const uint32_t n = min(Q.cnt[level], Q.cap);
+    const uint32_t m = Q.cnt[kFixHits + level];
+    if (m == 0) return;
+    const bool scan = m > kFixCap;
+    const uint32_t n = scan ? min(Q.cnt[level], Q.cap) : m;
     TraversalCounters cnt;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (Q.hit[i].x != kFixId) continue;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = scan ? j : Q.fix[j];
+        if (scan && Q.hit[i].x != kFixId) continue;
         const float4 a = Q.o[buf][i], b = Q.d[buf][i];
         float r = FLT_MAX, s = 0.f, t = 0.f;
         const uint32_t id = traverse<kRStack, EXH, false, false, false>(
@@ -427,7 +444,8 @@ __global__ __launch_bounds__(kRBlock) void k_fix_hits(RenderArgs A, Queues Q, in
 }
 
 // Shadow rays (any hit up to tmax).  REF: the refined tree; a ray with a zero
-// direction component is marked (sign bit of its tmax) for k_fix_shadows.
+// direction component is marked (sign bit of its tmax) and listed for
+// k_fix_shadows.
 template <bool EXH, bool REF>
 __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
     RenderArgs A, Queues Q, int level) {
@@ -450,6 +468,8 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
             if (REF && needs_reference(T.d)) {
                 float* w = reinterpret_cast<float*>(Q.sd[sb] + k) + 3;
                 *w = __uint_as_float(__float_as_uint(*w) | 0x80000000u);
+                const uint32_t slot = atomicAdd(Q.cnt + kFixShadows + level, 1u);
+                if (slot < kFixCap) Q.fix[(1 + sb) * kFixCap + slot] = k;
                 return;
             }
             if (!traversed || T.best_id == kInvalidId) shadow_lit(A, Q, sb, k);
@@ -459,14 +479,19 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
 }
 
 // The shadow rays of one level that k_shadow<..., true> marked, tested one
-// per lane on the reference tree.
+// per lane on the reference tree: the fix list's entries, or every marked
+// entry of the queue when the list overflowed.
 template <bool EXH>
 __global__ __launch_bounds__(kRBlock) void k_fix_shadows(RenderArgs A, Queues Q, int level) {
     __shared__ uint2 stk[kRStack * kRBlock];
-    const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
+    const uint32_t m = Q.cnt[kFixShadows + level];
+    if (m == 0) return;
+    const bool scan = m > kFixCap;
     const int sb = level & 1;
+    const uint32_t n = scan ? min(Q.cnt[kMaxLevels + level], Q.scap) : m;
     TraversalCounters cnt;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = scan ? j : Q.fix[(1 + sb) * kFixCap + j];
         const float4 b = Q.sd[sb][i];
         const uint32_t wbits = __float_as_uint(b.w);
         if (!(wbits >> 31)) continue;
@@ -1101,7 +1126,8 @@ size_t carve(Queues& q, ChunkState** cs, char* base, uint32_t cap) {
         q.sc[k] = (float4*)take((size_t)q.scap * 16);
         q.sx[k] = (float4*)take((size_t)q.scap * 16);
     }
-    q.cnt = (uint32_t*)take((4 * kMaxLevels + 2) * sizeof(uint32_t));
+    q.cnt = (uint32_t*)take((6 * kMaxLevels + 2) * sizeof(uint32_t));
+    q.fix = (uint32_t*)take(3 * (size_t)kFixCap * sizeof(uint32_t));
     q.xwork = (uint32_t*)take(2 * kMaxLevels * kXcds * 16 * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
     *cs = (ChunkState*)take(sizeof(ChunkState));
@@ -1463,7 +1489,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         const bool shadows = renderer != TN_RENDERER_RAYCASTER;
         int buf = 0;
         const bool refd = G.refined;
-        const dim3 fgrid((uint32_t)G.cus * 4u);
+        const dim3 fgrid(32u);  // fix lists are short (a scan after an overflow is rare)
         for (int lv = 0; lv < levels; ++lv) {
             if (refd) {
                 if (exh) k_trace<true, true><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);

@@ -671,8 +671,6 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
             const float3 rd = make_float3(__shfl(T.d.x, own, 64), __shfl(T.d.y, own, 64),
                                           __shfl(T.d.z, own, 64));
             const float bnd = __shfl(bound_own, own, 64);
-            uint32_t best_own = 0;
-            if constexpr (TIE) best_own = (uint32_t)__shfl((int)T.best_id, own, 64);
             if (item < total) {
                 const size_t r = rb + item;  // reference slot of the pair
                 float rr, ss, tt;
@@ -683,14 +681,10 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                     cnt.tests++;
                     cnt.full += code >= 1 ? 1u : 0u;
                 }
-                // a hit at exactly the current best distance ties only when it
-                // is another triangle (a triangle spanning several leaves is
-                // met again in later ones)
-                if (TIE && code == 3 && sc.refs[r] == best_own) code = 1;
                 if (code >= 2) {
                     // r >= 0 here; -0 maps to +0 so the key orders like the reference's <
                     const uint32_t rbits = __float_as_uint(rr + 0.0f);
-                    if (code == 2) atomicMin(&wl->slot[own], ((unsigned long long)rbits << 32) | item);
+                    atomicMin(&wl->slot[own], ((unsigned long long)rbits << 32) | item);
                     if constexpr (TIE) atomicMax(&wl->slot2[own], ((unsigned long long)(~rbits) << 32) | item);
                 }
             }
@@ -701,7 +695,15 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
         if constexpr (TIE) key2 = wl->slot2[lane];
         // every active lane moves on after its leaf, empty leaves included
         if (ray != kNone) {
-            if (lcnt && key != ~0ull) {
+            // TIE: the leaf's hits lie only at exactly the current best distance
+            // (code 3): the best stays; tied iff one of them is another
+            // triangle -- two or more of them (a triangle occurs once per
+            // leaf), or one that is not the best's (a triangle spanning
+            // several leaves is met again in later ones)
+            if (TIE && lcnt && key != ~0ull && T.best_id != kInvalidId &&
+                (uint32_t)(key >> 32) == __float_as_uint(T.best_r + 0.0f)) {
+                if ((uint32_t)key != (uint32_t)key2 || sc.refs[rbase + (uint32_t)key] != T.best_id) T.tie = true;
+            } else if (lcnt && key != ~0ull) {
                 const size_t r = rbase + (uint32_t)key;
                 const uint32_t id = sc.refs[r];
                 float rr, ss, tt;
@@ -719,8 +721,6 @@ __device__ __forceinline__ void coop_loop(const SceneView& sc, uint2* stk, int s
                     store(ray, T, true);
                     ray = kNone;
                 }
-            } else if (TIE && lcnt && key2 != 0ull) {
-                T.tie = true;  // a hit at exactly the current best distance
             }
             if (ray != kNone) {
                 bool done = trav_after_leaf<S, EXH, ANY>(T, tmx, stk, stride, cnt);
