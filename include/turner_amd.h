@@ -63,8 +63,10 @@ typedef struct {
  * prices a node step at 60 instead of 15 (a wave tests a leaf's triangles
  * together, so shallower trees with larger leaves render faster).  Closest hits
  * equal the reference tree's except between exactly equal distances (visit
- * order differs); the scene has no reference FlatNode array
- * (tn_scene_get_flat_nodes fails).  The reference tree stays the default. */
+ * order differs) and, rarely (of order one ray in 10^7), where the
+ * thick-plane clip leaves a sliver of a triangle out of a leaf the ray
+ * crosses; the scene has no reference FlatNode array (tn_scene_get_flat_nodes
+ * fails).  The reference tree stays the default. */
 #define TN_SCENE_EMPTY_LEAVES 2u
 /* Refined reference tree, for renders whose hit ids are the reference's.  The
  * scene keeps the reference tree (tn_intersect, tn_scene_get_flat_nodes) and
@@ -72,7 +74,8 @@ typedef struct {
  * order, each leaf's references filtered to triangles whose bounding box
  * reaches the leaf (the others were sunk into it by the reference's
  * unbounded-box quirk) and larger leaves split further by rendering-variant
- * sub-trees.  Renders (and tn_intersect with TN_ISECT_REFINED) traverse the
+ * sub-trees whose every leaf also holds each of those triangles that meets its
+ * cell.  Renders (and tn_intersect with TN_ISECT_REFINED) traverse the
  * refined tree, detect every ray whose closest hit ties with another triangle
  * at exactly the same distance -- only there can the order inside a leaf
  * decide the id -- and re-trace those rays, and rays with a zero direction

@@ -143,3 +143,21 @@ def test_refined_render_equals_reference_render(gpu):
         assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"]), (renderer, flags)
         np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5, err_msg=f"renderer {renderer} flags {flags}")
         print(f"[refined render] renderer {renderer} flags {flags}: {sa['rays']} segments, equal", flush=True)
+
+
+def test_refined_render_with_duplicated_triangles(gpu):
+    """Every triangle of the Cornell box twice (same geometry, two ids): every
+    hit is an exact tie, so every segment goes to the fix list and is traced
+    on the reference tree by the last block of the trace kernel; the render
+    equals the reference tree's (same winners, the first copy in leaf order)."""
+    sc = scenes.cornell_box()
+    dup = {k: np.concatenate([sc[k], sc[k]]) for k in ("verts", "normals", "albedo", "emission")}
+    ref = T.KDTree(dup["verts"], dup["normals"], dup["albedo"], dup["emission"])
+    fin = T.KDTree(dup["verts"], dup["normals"], dup["albedo"], dup["emission"], refined=True)
+    conf = T.TracerConfig(width=128, max_recursion_depth=3, num_pixel_samples=8, num_monte_carlo_samples=2,
+                          flags=TN_RENDER_REF_JITTER, seed=9)
+    a, sa = _render_sums(ref, sc, conf)
+    b, sb = _render_sums(fin, sc, conf)
+    assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"])
+    np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5)
+    print(f"[refined render] duplicated triangles: {sa['rays']} segments, equal", flush=True)
