@@ -149,7 +149,17 @@ int tn_tree_cache_load(const char* path, const float* verts9, uint32_t num_trian
                        uint64_t** flat_nodes, size_t* num_nodes, float* box6);
 
 /* Intersection flags */
-#define TN_ISECT_EXHAUSTIVE 1u /* visit every leaf the ray crosses (reference order) */
+/* Visit every leaf the ray crosses, in the reference's order, as
+ * KDTreeIntersection::intersect does (kdtree.cpp:515-607).  Without it a ray
+ * stops after the first leaf whose exit lies beyond its best hit by a 2^-16
+ * margin: the same hits on scenes of about unit scale (checked on every test
+ * and benchmark batch), but not always where coordinates are far larger than
+ * the triangles (a mesh of unit size moved 1e2..1e4 from the origin: a few in
+ * 10^4 of rays aimed at vertices) or where the clipping tolerance 1e-5 is not
+ * small against the triangles (a unit mesh scaled by 1e-3): there a later
+ * leaf can still hold a triangle hit a little before the best one.  Use this
+ * flag for such scenes. */
+#define TN_ISECT_EXHAUSTIVE 1u
 #define TN_ISECT_MT 2u         /* FP32 Moeller-Trumbore leaf test instead of the reference's */
 #define TN_ISECT_COUNT 4u      /* accumulate work counters (tn_get_counters) */
 /* Traversal schedules (default: wave-cooperative leaf tests -- the wave pools

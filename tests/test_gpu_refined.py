@@ -35,18 +35,24 @@ def _occ(tree, to, td, tmax, flags):
     return occ
 
 
-def _same(tree, o, d, what):
-    """refined == default (ids, r, s, t) and occlusion at the hit distance, half of it and beyond"""
+def _same(tree, o, d, what, exhaustive=False):
+    """refined == default (ids, r, s, t) and occlusion at the hit distance, half
+    of it and beyond; refined exhaustive == reference exhaustive, and (unless
+    `exhaustive`: scenes far from unit scale, where the near-first early exit
+    is not the reference's) == default"""
     import torch
     to, td = torch.from_numpy(np.ascontiguousarray(o, np.float32)).cuda(), \
         torch.from_numpy(np.ascontiguousarray(d, np.float32)).cuda()
     a = _isect(tree, to, td, 0)
     b = _isect(tree, to, td, TN_ISECT_REFINED)
     e = _isect(tree, to, td, TN_ISECT_REFINED | TN_ISECT_EXHAUSTIVE)
+    x2 = _isect(tree, to, td, TN_ISECT_EXHAUSTIVE)
     torch.cuda.synchronize()
-    for x, y, z in zip(a, b, e):
+    for x, y, z, w in zip(a, b, e, x2):
         assert torch.equal(x, y), f"{what}: refined differs"
-        assert torch.equal(x, z), f"{what}: refined exhaustive differs"
+        assert torch.equal(w, z), f"{what}: refined exhaustive differs from reference exhaustive"
+        if not exhaustive:
+            assert torch.equal(x, z), f"{what}: refined exhaustive differs"
     r = a[1].clone()
     for tm in (r, r * 0.5, torch.where(r < 1e30, r * 1.5, r)):
         tm = tm.contiguous()
@@ -67,6 +73,36 @@ def test_refined_matches_reference_small_scenes(gpu):
         o2 = rng.uniform(lo - 0.5, hi + 0.5, (1 << 16, 3))
         d2 = rng.normal(size=(1 << 16, 3))
         _same(tree, np.concatenate([o, o2]), np.concatenate([d, d2]), name)
+
+
+def test_refined_matches_reference_across_scales(gpu):
+    """The refined tree's tolerances scale with the coordinates' magnitude: the
+    same meshes scaled by 1e-3 and 1e3 and moved 1e4 away from the origin
+    (where an ulp is 2^-10) keep the reference tree's hits, with the near-first
+    early exit on both trees and with the exhaustive traversal on both; the
+    exhaustive traversal is the reference's (oracle) on a sample."""
+    import oracle as O
+    rng = np.random.default_rng(21)
+    base = [("icosphere level 4", scenes.intersect_scene(level=4)["verts"]), ("soup", H.random_soup(2000, rng)),
+            ("grid", H.grid_mesh(16))]
+    for name, v0 in base:
+        for scale, shift in ((1e-3, 0.0), (1e3, 0.0), (1.0, 1e4)):
+            v = (v0.astype(np.float64) * scale + shift).astype(np.float32)
+            tree = T.KDTree(v, refined=True)
+            o, d = H.hard_rays(v, 1 << 15, rng)
+            lo, hi = v.reshape(-1, 3).min(0), v.reshape(-1, 3).max(0)
+            o2 = rng.uniform(lo - 0.3 * (hi - lo), hi + 0.3 * (hi - lo), (1 << 15, 3))
+            d2 = rng.normal(size=(1 << 15, 3))
+            o, d = np.concatenate([o, o2]).astype(np.float32), np.concatenate([d, d2]).astype(np.float32)
+            _same(tree, o, d, f"{name} x{scale:g} +{shift:g}", exhaustive=True)
+            m = 1 << 13
+            oid, orr, _, _ = O.intersect(tree.triangle_records(), tree.flat_nodes(), tree.box(), o[:m], d[:m])
+            import torch
+            ids, r, _, _ = _isect(tree, torch.from_numpy(o[:m]).cuda(), torch.from_numpy(d[:m]).cuda(),
+                                  TN_ISECT_REFINED | TN_ISECT_EXHAUSTIVE)
+            assert np.array_equal(ids.cpu().numpy().view(np.uint32), oid), name
+            assert np.array_equal(r.cpu().numpy().view(np.uint32), orr.view(np.uint32)), name
+            tree.close()
 
 
 @pytest.mark.slow
