@@ -208,3 +208,10 @@ def test_cli_raycaster_and_raytracer_match_oracle(gpu, tmp_path):
         assert diff.max() <= 1 and np.mean(diff == 0) > 0.97
         stats = dict(re.findall(r"^(\w[\w ()/-]*?)\s*: (\S+)", p.stderr, re.M))
         assert int(stats["Rays"]) == nr
+        # the default renders on the refined copy of the reference tree: the
+        # plain reference tree gives the same image, bit for bit
+        pfm2 = str(tmp_path / (os.path.basename(exe) + "_ref.pfm"))
+        q = run_exe(exe, path, "-w", str(w), "--background", "0.1 0.2 0.3", "--pfm", pfm2, "--reference-tree", *extra)
+        assert q.returncode == 0, q.stderr
+        assert q.stdout == p.stdout
+        assert np.array_equal(T.read_pfm(pfm2), T.read_pfm(pfm))

@@ -59,6 +59,8 @@ MI355X options:
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
   --render-tree                     Rendering kd-tree (drops the reference builder's fully clipped
                                     triangles and empty-side quirks; same closest hits).
+  --reference-tree                  The reference kd-tree as built, without the refined copy of
+                                    it that renders by default (same hits, about 3x slower).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
   --guided                          Learned path guiding (needs -m 1): trains online, then renders.
   --train-passes=<int>              Guiding: training passes of 2 spp [default: 16].
@@ -76,6 +78,7 @@ MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per ro
   --exhaustive                      Reference traversal order without early exit.
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
   --render-tree                     Rendering kd-tree (same closest hits).
+  --reference-tree                  The reference kd-tree without its refined copy (same hits).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
   --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
@@ -92,6 +95,7 @@ MI355X options (the pixel jitter is the reference's: xorshift64*, seed 42 per ro
   --exhaustive                      Reference traversal order without early exit.
   --tree-cache=<path>               kd-tree cache file (validated against the scene).
   --render-tree                     Rendering kd-tree (same closest hits).
+  --reference-tree                  The reference kd-tree without its refined copy (same hits).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
   --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
@@ -118,7 +122,7 @@ const std::map<std::string, std::string> kShort = {
     {"-p", "--pixel-samples"}, {"-m", "--monte-carlo-samples"}, {"-v", "--verbose"}};
 const char* kFlags[] = {"--no-gamma-correction", "--verbose", "--cosine", "--area-lights",
                         "--ref-jitter", "--exhaustive", "--print-config", "--help",
-                        "--render-tree", "--graph", "--guided"};
+                        "--render-tree", "--reference-tree", "--graph", "--guided"};
 
 bool is_flag(const std::string& name) {
     for (const char* f : kFlags)
@@ -179,7 +183,7 @@ std::string get(const Args& a, const std::string& k, const std::string& def) {
 int main(int argc, char** argv) {
     using namespace turner_amd;
     TracerConfig conf;
-    bool render_tree = false, guided = false;
+    bool render_tree = false, reference_tree = false, guided = false;
     std::string pfm;
     int train_passes = 16;
     try {
@@ -215,6 +219,7 @@ int main(int argc, char** argv) {
         if (a.opt.count("--exhaustive")) conf.flags |= TN_RENDER_EXHAUSTIVE;
         if (a.opt.count("--graph")) conf.flags |= TN_RENDER_GRAPH;
         render_tree = a.opt.count("--render-tree") > 0;
+        reference_tree = a.opt.count("--reference-tree") > 0;
         pfm = get(a, "--pfm", "");
         guided = kRenderer == TN_RENDERER_PATHTRACER && a.opt.count("--guided") > 0;
         train_passes = std::stoi(get(a, "--train-passes", "16"));
@@ -251,9 +256,13 @@ int main(int argc, char** argv) {
         float aspect = sc.camera.aspect > 0 ? sc.camera.aspect : conf.aspect;
         std::cerr << "Loading triangles and building kd-tree..." << std::endl;
         const size_t ntri = sc.triangles.size();
-        KDTree tree = render_tree ? KDTree(std::move(sc.triangles), (uint32_t)TN_SCENE_EMPTY_LEAVES)
-                      : conf.tree_cache.empty() ? KDTree(std::move(sc.triangles))
-                                                : KDTree(std::move(sc.triangles), conf.tree_cache);
+        // default: the reference tree with its refined copy (TN_SCENE_REFINED),
+        // whose renders give the reference tree's hits bit for bit; a cached
+        // tree is the reference tree as stored
+        KDTree tree = render_tree      ? KDTree(std::move(sc.triangles), (uint32_t)TN_SCENE_EMPTY_LEAVES)
+                      : !conf.tree_cache.empty() ? KDTree(std::move(sc.triangles), conf.tree_cache)
+                      : reference_tree ? KDTree(std::move(sc.triangles))
+                                       : KDTree(std::move(sc.triangles), (uint32_t)TN_SCENE_REFINED);
         auto t1 = std::chrono::steady_clock::now();
         std::cerr << "KDTree runtime: " << (size_t)tree.build_ms() << std::endl;
         const size_t height = (size_t)(conf.width / aspect);
