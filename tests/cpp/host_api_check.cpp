@@ -15,9 +15,10 @@
 // origins, n x 3 float directions.
 //
 // usage:
-//   host_api_check intersect <scene.tns> <rays.bin> <out.bin> <singles>
+//   host_api_check intersect <scene.tns> <rays.bin> <out.bin> <singles> [scene_flags]
 //       out: n uint32 ids, n x 3 float (r, s, t) batched; then `singles` x
-//       (uint32 id, float r, a, b) of the one-ray calls (r, a, b start as NaN)
+//       (uint32 id, float r, a, b) of the one-ray calls (r, a, b start as NaN);
+//       scene_flags: TN_SCENE_* for KDTree(triangles, flags) (default: KDTree(triangles))
 //   host_api_check trace <scene.tns> <rays.bin> <out.bin> <singles> <depth> <max_depth>
 //                  <mc> <flags> <seed>
 //       out: n x RGBA batched, then `singles` x RGBA of one-ray calls
@@ -70,9 +71,11 @@ int run(int argc, char** argv) {
     }
     const std::string mode = argv[1];
     Scene sc = load_tns(argv[2]);
-    KDTree tree(sc.triangles);
+    const uint32_t scene_flags =
+        (mode == "intersect" && argc == 7) ? (uint32_t)std::strtoul(argv[6], nullptr, 10) : 0u;
+    KDTree tree = scene_flags ? KDTree(sc.triangles, scene_flags) : KDTree(sc.triangles);
     KDTreeIntersection ti(tree);
-    if (mode == "intersect" && argc == 6) {
+    if (mode == "intersect" && (argc == 6 || argc == 7)) {
         const std::vector<Ray> rays = read_rays(argv[3]);
         const size_t singles = std::min<size_t>(std::strtoul(argv[5], nullptr, 10), rays.size());
         std::vector<uint32_t> ids;

@@ -154,9 +154,11 @@ def run_check(*args):
     assert p.returncode == 0, p.stdout
 
 
-def test_cpp_intersect_matches_oracle(gpu, tmp_path):
+@pytest.mark.parametrize("scene_flags", [0, 4])
+def test_cpp_intersect_matches_oracle(gpu, tmp_path, scene_flags):
     """KDTreeIntersection::intersect from C++ (one ray per call and batched)
-    against the oracle's tree, precompute and traversal, bit for bit."""
+    against the oracle's tree, precompute and traversal, bit for bit; with
+    scene flags 4 (TN_SCENE_REFINED) the calls walk the refined tree."""
     sc = scenes.intersect_scene(level=4)
     nt = len(sc["verts"])
     sc.update(camera=scenes.camera((0, 0, 0), 60.0), albedo=np.tile(np.float32([0.8, 0.8, 0.8, 1]), (nt, 1)),
@@ -170,7 +172,7 @@ def test_cpp_intersect_matches_oracle(gpu, tmp_path):
     o, d = np.concatenate([mo, o, ho]), np.concatenate([md, d, hd])
     write_rays(tmp_path / "r.bin", o, d)
     singles = 512
-    run_check("intersect", tmp_path / "s.tns", tmp_path / "r.bin", tmp_path / "o.bin", singles)
+    run_check("intersect", tmp_path / "s.tns", tmp_path / "r.bin", tmp_path / "o.bin", singles, scene_flags)
     raw = np.fromfile(tmp_path / "o.bin", np.uint32)
     n = len(o)
     ids, r, s, t = raw[:n], raw[n:2 * n].view(np.float32), raw[2 * n:3 * n].view(np.float32), \

@@ -199,8 +199,12 @@ class KDTreeIntersection:
     """lib/kdtree.h:270-309.  intersect() for one ray mirrors the reference's
     out-parameter call; intersect_batch() is the batched GPU twin."""
 
-    def __init__(self, tree, flags=0):
+    def __init__(self, tree, flags=None):
+        """flags: TN_ISECT_*; by default the refined traversal on a tree built
+        with refined=True (the reference tree's hits, faster), else 0."""
         self.tree = tree
+        if flags is None:
+            flags = _lib.TN_ISECT_REFINED if tree.info.refined_nodes else 0
         self.flags = flags
 
     def intersect(self, o, d):

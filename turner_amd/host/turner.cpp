@@ -180,7 +180,8 @@ bool enters_root_box(const Ray& ray, const float* box) {
 OptionalId KDTreeIntersection::intersect(const Ray& ray, float& r, float& a, float& b) {
     uint32_t id;
     float rr, ss, tt;
-    check(tn_intersect_host(tree_->handle(), &ray.o.x, &ray.d.x, 1, &id, &rr, &ss, &tt, 0));
+    check(tn_intersect_host(tree_->handle(), &ray.o.x, &ray.d.x, 1, &id, &rr, &ss, &tt,
+                            tree_->refined() ? TN_ISECT_REFINED : 0u));
     if (id == TN_INVALID_ID) {
         // the reference sets r to FLT_MAX once the ray enters the root box
         // (lib/kdtree.cpp:535) and leaves it there on a miss
@@ -207,7 +208,7 @@ void KDTreeIntersection::intersect(const std::vector<Ray>& rays, std::vector<uin
     s.resize(n);
     t.resize(n);
     check(tn_intersect_host(tree_->handle(), o.data(), d.data(), (uint32_t)n, ids.data(), r.data(),
-                            s.data(), t.data(), 0));
+                            s.data(), t.data(), tree_->refined() ? TN_ISECT_REFINED : 0u));
 }
 
 std::vector<Color> trace(const std::vector<Ray>& rays, KDTreeIntersection& ti,

@@ -91,6 +91,8 @@ public:
     const Triangle& at(uint32_t id) const { return tris_.at(id); }
     tn_scene* handle() const { return scene_; }
     double build_ms() const { return info_.build_ms; }
+    // built with TN_SCENE_REFINED: intersect() walks the refined tree
+    bool refined() const { return info_.refined_nodes != 0; }
 
 private:
     void create(const uint64_t* nodes, size_t n, const float* box, uint32_t flags = 0);
