@@ -244,6 +244,10 @@ def main():
                                if traffic is not None else None),
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel": "k_rays_coop<closest hit, refined tree> (tn_intersect, TN_ISECT_REFINED)",
+            "note": ("the refined tree's traversal reads %.0f algorithmic bytes per ray against the reference "
+                     "tree's %.0f (reference_tree_traversal), so it reaches a lower fraction of the HBM roofline "
+                     "at %.2fx the rays/s: its launch is bound by the latency of dependent node fetches"
+                     % (alg_bytes / n, ref_bytes / n, value / (n / (ref_ms * 1e-3) / 1e6))),
         },
         "reference_tree_traversal": {
             "value": round(n / (ref_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "ms_per_step": round(ref_ms, 4),
