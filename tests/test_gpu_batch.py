@@ -70,6 +70,27 @@ def test_batch_matches_single_renders(gpu):
     b.close()
 
 
+def test_batch_of_refined_scenes_matches_single_renders(gpu):
+    """Scenes built with TN_SCENE_REFINED: the batch traverses their
+    reference trees, their single renders the refined trees; both give the
+    reference tree's hits, so the images agree as for any batch."""
+    from turner_amd.batch import Batch
+    scs = _scenes()
+    trees = [T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], refined=True) for sc in scs]
+    jobs = _jobs(scs)
+    b = Batch(trees)
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    imgs, st, _ = b.render(jobs, max_depth=2, mc_samples=2, flags=flags, seed=3)
+    rays = 0
+    for s, (tree, sc, job) in enumerate(zip(trees, scs, jobs)):
+        ref, sst = _single(tree, sc, job, 2, 2, flags, (3 + s * STEP) & MASK)
+        np.testing.assert_allclose(imgs[s].cpu().numpy() * job["spp"], ref, rtol=1e-5, atol=1e-6,
+                                   err_msg=f"scene {s}")
+        rays += sst["rays"]
+    assert st["rays"] == rays
+    b.close()
+
+
 def test_graph_replay_matches_direct(gpu):
     sc = scenes.sphere_in_cornell(level=3)
     tree = _tree(sc)

@@ -10,7 +10,7 @@ import pytest
 import turner_amd as T
 from turner_amd import scenes
 from turner_amd._lib import (TN_ISECT_EXHAUSTIVE, TN_ISECT_REFINED, TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE,
-                             TN_RENDER_REF_JITTER)
+                             TN_RENDER_EXHAUSTIVE, TN_RENDER_REF_JITTER)
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
@@ -179,6 +179,20 @@ def test_refined_render_equals_reference_render(gpu):
         assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"]), (renderer, flags)
         np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5, err_msg=f"renderer {renderer} flags {flags}")
         print(f"[refined render] renderer {renderer} flags {flags}: {sa['rays']} segments, equal", flush=True)
+
+
+def test_refined_exhaustive_render_equals_reference(gpu):
+    """TN_RENDER_EXHAUSTIVE (every leaf along every segment, the reference's
+    traversal) on the refined tree equals it on the reference tree."""
+    sc = scenes.sphere_in_cornell(level=5)
+    ref = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    fin = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], refined=True)
+    conf = T.TracerConfig(width=64, max_recursion_depth=3, num_pixel_samples=8, num_monte_carlo_samples=2,
+                          flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS | TN_RENDER_EXHAUSTIVE, seed=4)
+    a, sa = _render_sums(ref, sc, conf)
+    b, sb = _render_sums(fin, sc, conf)
+    assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"])
+    np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5)
 
 
 def test_refined_render_with_duplicated_triangles(gpu):
