@@ -49,6 +49,8 @@ def _same(tree, o, d, what, exhaustive=False):
     x2 = _isect(tree, to, td, TN_ISECT_EXHAUSTIVE)
     torch.cuda.synchronize()
     for x, y, z, w in zip(a, b, e, x2):
+        # bit for bit (a hit on a sliver whose barycentric denominator is 0 has s = t = NaN on both)
+        x, y, z, w = (q.view(torch.int32) for q in (x, y, z, w))
         assert torch.equal(x, y), f"{what}: refined differs"
         assert torch.equal(w, z), f"{what}: refined exhaustive differs from reference exhaustive"
         if not exhaustive:
@@ -73,6 +75,42 @@ def test_refined_matches_reference_small_scenes(gpu):
         o2 = rng.uniform(lo - 0.5, hi + 0.5, (1 << 16, 3))
         d2 = rng.normal(size=(1 << 16, 3))
         _same(tree, np.concatenate([o, o2]), np.concatenate([d, d2]), name)
+
+
+def test_refined_edge_cases(gpu):
+    """Scenes the refinement leaves alone or splits into tiny sub-trees: one
+    triangle, four separated triangles, slivers (a third vertex 1e-4 off the
+    line of the other two; degenerate triangles are rejected at creation, as
+    the reference asserts) among regular ones, two coincident copies of every
+    triangle; rays with zero, huge and tiny direction components, NaN
+    directions and origins on triangles."""
+    import torch
+    rng = np.random.default_rng(31)
+    soup = H.random_soup(300, rng)
+    thin = soup.copy()
+    thin[:100, 6:9] = 2 * thin[:100, 3:6] - thin[:100, 0:3] + np.float32(1e-4)
+    cases = [("one", H.tri((0, 0, 1), (0, 1, 1), (1, 0, 1))[None]), ("four", H.four_separated()),
+             ("slivers", thin), ("coincident", np.concatenate([soup, soup]))]
+    for name, v in cases:
+        v = np.ascontiguousarray(v, np.float32)
+        tree = T.KDTree(v, refined=True)
+        o, d = H.hard_rays(v, 1 << 12, rng)
+        k = len(o) // 4
+        d[:k] *= np.float32(1e6)
+        d[k:2 * k] *= np.float32(1e-6)
+        pts = v.reshape(-1, 3, 3)[rng.integers(0, len(v), k)].mean(1)
+        o[2 * k:3 * k] = pts  # origins on triangles
+        _same(tree, o, d, f"edge {name}")
+        # NaN directions end without a hit on both trees
+        dn = d.copy()
+        dn[:64] = np.nan
+        to, td = torch.from_numpy(o).cuda(), torch.from_numpy(np.ascontiguousarray(dn)).cuda()
+        a = _isect(tree, to, td, 0)
+        b = _isect(tree, to, td, TN_ISECT_REFINED)
+        torch.cuda.synchronize()
+        for x, y in zip(a, b):
+            assert torch.equal(x[:64].view(torch.int32), y[:64].view(torch.int32)), name
+        tree.close()
 
 
 def test_refined_matches_reference_across_scales(gpu):
