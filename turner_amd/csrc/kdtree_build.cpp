@@ -495,10 +495,8 @@ void precompute_triangles(const float* verts9, uint32_t n, float* out16) {
 // whose thick-plane clipping can drop a sliver beyond a split plane, a
 // triangle whose clipped box is thinner than the clipping tolerance, or one
 // whose vertex lies on a split plane: a hit point inside a cell is then found
-// in that cell.  Leaf order: ascending triangle id.  Subtrees with many
-// candidates run on threads of their own when parallel.
-void cover_tree(const float* verts9, Node* n, const Box& box, const std::vector<uint32_t>& cand, float tol,
-                bool parallel) {
+// in that cell.  Leaf order: ascending triangle id.
+void cover_tree(const float* verts9, Node* n, const Box& box, const std::vector<uint32_t>& cand, float tol) {
     std::vector<uint32_t> mine;
     mine.reserve(cand.size());
     for (uint32_t id : cand) {
@@ -508,15 +506,8 @@ void cover_tree(const float* verts9, Node* n, const Box& box, const std::vector<
     if (!n->leaf) {
         Box l, r;
         split(box, n->axis, n->split, l, r);
-        if (parallel && mine.size() > 65536) {
-            auto fut = std::async(std::launch::async,
-                                  [&]() { cover_tree(verts9, n->left.get(), l, mine, tol, true); });
-            cover_tree(verts9, n->right.get(), r, mine, tol, true);
-            fut.get();
-        } else {
-            cover_tree(verts9, n->left.get(), l, mine, tol, parallel);
-            cover_tree(verts9, n->right.get(), r, mine, tol, parallel);
-        }
+        cover_tree(verts9, n->left.get(), l, mine, tol);
+        cover_tree(verts9, n->right.get(), r, mine, tol);
         return;
     }
     n->ids.insert(n->ids.end(), mine.begin(), mine.end());
@@ -682,7 +673,7 @@ void refine_reference_tree(const float* verts9, const std::vector<DevNode>& rnod
         if (keep.size() > 4) {
             Builder B(verts9, false, true, tol, kRefinedTrav);
             j.sub = B.build(keep, j.box);
-            cover_tree(verts9, j.sub.get(), j.box, keep, std::ldexp(mag, -26), false);
+            cover_tree(verts9, j.sub.get(), j.box, keep, std::ldexp(mag, -26));
             j.ids.clear();
         } else {
             j.ids = std::move(keep);
