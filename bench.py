@@ -344,7 +344,9 @@ def batch_throughput(args):
     return {
         "workload": ("32 scenes in one batch: white furnace + 31 displaced icospheres (levels 3..8) in "
                      "Cornell enclosures, widths 128..1024, 16..64 spp, max-depth 3, m 1, cosine + "
-                     "area-light NEE, rendering kd-trees, graph-replayed chunks, autotuned shape"),
+                     "area-light NEE, rendering kd-trees, autotuned shape: "
+                     + ("graph-replayed chunks, one pipeline" if tune["graph"] else
+                        "direct launches, two pipelines (faster than graph replay on this batch)")),
         "scenes": len(scs), "levels": levels, "triangles": int(sum(len(sc["verts"]) for sc in scs)),
         "samples": samples, "host_build_s": round(build_s, 2), "seconds": round(sec, 3),
         "closest_hit_mrays_s": round(st["rays"] / sec / 1e6, 1),
