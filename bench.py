@@ -73,6 +73,9 @@ def main():
                     help="skip the batched heterogeneous rendering measurement")
     ap.add_argument("--guided-ref-spp", type=int, default=8192)
     ap.add_argument("--guided-width", type=int, default=512)
+    ap.add_argument("--guided-min-pass", type=int, default=8,
+                    help="image passes after the first are sized from the error so far (relMSE ~ 1/spp), "
+                         "in multiples of this many spp, at most 32 (0: fixed 32-spp passes)")
     ap.add_argument("--render-spp", type=int, default=16)
     ap.add_argument("--render-repeats", type=int, default=3)
     ap.add_argument("--mlp-probe", action="store_true",
@@ -705,6 +708,7 @@ def guided_vs_plain(args):
     included in its time), plus the MFMA rate of the guide kernels."""
     import ctypes as C
 
+    import numpy as np
     import torch
 
     import turner_amd as T
@@ -730,10 +734,12 @@ def guided_vs_plain(args):
     T.render_accumulate(tree, sc["camera"], sc["lights"], tiny, scratch)
     render_guided(tree, sc["camera"], sc["lights"], tiny, g, scratch, train_samples=1 << 14)
     g.close()
-    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=32)
+    plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=32,
+                           min_pass=args.guided_min_pass)
     g = Guide.for_tree(tree, seed=2, epochs=4)
     guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12, ref_var=ref_var, spp_per_pass=32,
-                            train_passes=8, train_spp=1, train_samples=1 << 18, keep_training_passes=False)
+                            train_passes=8, train_spp=1, train_samples=1 << 18, keep_training_passes=False,
+                            min_pass=args.guided_min_pass)
     ginfo = g.info()
     # MFMA rate of the guide kernels (HIP events on the launch stream)
     dev = torch.device("cuda:0")
@@ -764,19 +770,37 @@ def guided_vs_plain(args):
     fwd_tf = fwd_flop * npts / (fwd_ms * 1e-3) / 1e12
     train_tf = 3 * fwd_flop * nb / (train_ms * 1e-3) / 1e12
     ratio = guided["seconds"] / plain["seconds"] if plain["reached"] and guided["reached"] else None
+    # two more seed pairs: the ratio moves with the 32-spp stopping granularity
+    seeds = [{"plain_s": round(plain["seconds"], 5), "plain_spp": plain["spp"], "guided_s": round(guided["seconds"], 5),
+              "guided_spp": guided["spp"], "ratio": None if ratio is None else round(ratio, 4)}]
+    for k in (1, 2):
+        pl = time_to_relmse(tree, sc, ref, depth, flags, seed=11 + 100 * k, ref_var=ref_var, spp_per_pass=32,
+                            min_pass=args.guided_min_pass)
+        g2 = Guide.for_tree(tree, seed=2 + 100 * k, epochs=4)
+        gu = time_to_relmse(tree, sc, ref, depth, flags, guide=g2, seed=12 + 100 * k, ref_var=ref_var,
+                            spp_per_pass=32, train_passes=8, train_spp=1, train_samples=1 << 18,
+                            keep_training_passes=False, min_pass=args.guided_min_pass)
+        g2.close()
+        rk = gu["seconds"] / pl["seconds"] if pl["reached"] and gu["reached"] else None
+        seeds.append({"plain_s": round(pl["seconds"], 5), "plain_spp": pl["spp"], "guided_s": round(gu["seconds"], 5),
+                      "guided_spp": gu["spp"], "ratio": None if rk is None else round(rk, 4)})
+    rs = [x["ratio"] for x in seeds if x["ratio"] is not None]
     return {
         "workload": (f"occluded-light Cornell box + level-8 icosphere, {w}x{w}, max-depth 6, cosine + "
                      f"area-light NEE, rendering kd-tree, relMSE target 0.01 vs a {args.guided_ref_spp}-spp "
                      f"plain reference ({ref_s:.1f} s; its own per-pixel variance, relMSE "
                      f"{ref_noise:.4f}, is subtracted)"),
         "guided_schedule": ("8 training passes of 1 spp, each training on a uniform 2^18-sample subset of "
-                            "its radiance records for 4 epochs (Adam lr 1e-3, batch 16384), then 32-spp "
+                            "its radiance records for 4 epochs (Adam lr 1e-3, batch 16384), then image "
                             "passes; training passes are timed but left out of the image; one-sample MIS "
-                            "0.5 guide / 0.5 cosine; both tracers add 32-spp passes and sample emitters "
-                            "by solid angle"),
+                            "0.5 guide / 0.5 cosine; both tracers add image passes of 32 spp, each after "
+                            "the first sized from the error so far (relMSE ~ 1/spp: the missing samples, "
+                            f"rounded up to a multiple of {args.guided_min_pass} spp, at most 32), and sample "
+                            "emitters by solid angle"),
         "plain": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items() if k != "history"},
         "guided": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in guided.items() if k != "history"},
         "guided_time_over_plain": None if ratio is None else round(ratio, 4),
+        "guided_time_over_plain_3_seeds": {"each": seeds, "mean": round(float(np.mean(rs)), 4) if rs else None},
         "guide_steps": ginfo["steps"], "guide_last_loss": round(ginfo["loss"], 4),
         "mfma": {"forward_tflops": round(fwd_tf, 2), "train_tflops": round(train_tf, 2),
                  "peak_bf16_tflops": 2500.0, "forward_frac": round(fwd_tf / 2500.0, 5),

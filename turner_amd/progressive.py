@@ -27,7 +27,7 @@ def _conf(width, height, depth, flags, spp, seed):
 def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=64,
                    train_passes=16, train_samples=1 << 18, train_spp=2, max_seconds=60.0, seed=1,
                    max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0,
-                   passes_per_render=1):
+                   passes_per_render=1, min_pass=0):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
     reference's per-pixel variance (relmse subtracts it).
@@ -39,7 +39,11 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
     image).  passes_per_render=p renders the data of p training passes in
     one render of p * train_spp samples per pixel and trains p passes on it
     (each on its own uniform subset of train_samples records); a small render
-    costs about as much as a larger one, its slowest rays bound it."""
+    costs about as much as a larger one, its slowest rays bound it.
+    min_pass=q > 0 sizes each image pass after the first from the error so
+    far: relMSE falls as 1/spp, so about spp * err / target samples reach the
+    target; the next pass takes the missing samples rounded up to a multiple
+    of q, at most spp_per_pass (the same rule for both tracers)."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -56,6 +60,9 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
                 and (k - train_renders) % train_every == 0)
         extra = min(ppr, train_passes - k * ppr) - 1 if training else 0  # more passes on this render's data
         n = train_spp * (extra + 1) if training else spp_per_pass
+        if not training and min_pass > 0 and spp_img > 0 and err > target and err != float("inf"):
+            missing = spp_img * err / target - spp_img
+            n = int(min(spp_per_pass, max(min_pass, -(-missing // min_pass) * min_pass)))
         conf = _conf(w, h, depth, flags, n, seed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
