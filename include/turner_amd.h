@@ -243,7 +243,15 @@ typedef struct {
 #define TN_RENDER_GUIDED 8u      /* learned guiding proposal (needs a tn_guide) */
 #define TN_RENDER_EXHAUSTIVE 16u /* reference traversal order without early exit */
 #define TN_RENDER_GRAPH 32u      /* capture the per-chunk launch sequence once in a hipGraph, replay per chunk */
-#define TN_RENDER_AUTOTUNE 64u   /* batches: chunk size, grid and graph replay picked by timing probes (cached) */
+#define TN_RENDER_AUTOTUNE 64u   /* batches: chunk size, grid and launch mode picked by timing probes (cached) */
+/* Plain path tracing (no guide, no training records, camera samples, not a
+ * TN_SCENE_REFINED scene) in one cooperative launch of a persistent kernel
+ * that walks every chunk and level, its stages separated by grid barriers
+ * instead of kernel boundaries (no reference counterpart).  Same paths and
+ * counts as the launch-per-stage sequence; slower on MI355X (one pipeline,
+ * no shadow/trace overlap, the union of the stages' registers), so the
+ * autotuner does not pick it.  Ignored where it does not apply. */
+#define TN_RENDER_PERSISTENT 128u
 
 typedef struct {
     uint32_t width, height;

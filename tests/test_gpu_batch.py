@@ -17,7 +17,8 @@ import pytest
 
 import turner_amd as T
 from turner_amd import scenes
-from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_AUTOTUNE, TN_RENDER_COSINE, TN_RENDER_GRAPH
+from turner_amd._lib import (TN_RENDER_AREA_LIGHTS, TN_RENDER_AUTOTUNE, TN_RENDER_COSINE, TN_RENDER_GRAPH,
+                             TN_RENDER_PERSISTENT)
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +90,35 @@ def test_batch_of_refined_scenes_matches_single_renders(gpu):
         rays += sst["rays"]
     assert st["rays"] == rays
     b.close()
+
+
+def test_persistent_launch_matches_wavefront(gpu):
+    """TN_RENDER_PERSISTENT: one cooperative launch walks every chunk and
+    level (grid barriers between the stages).  The same paths as the
+    launch-per-stage wavefront (equal segment and shadow-ray counts, pixels
+    within FP32 accumulation order), for single renders of several chunks
+    (m = 1 and m = 2, uniform and cosine sampling) and for a batch."""
+    from turner_amd.batch import Batch
+    sc = scenes.sphere_in_cornell(level=3)
+    tree = _tree(sc)
+    job = {"camera": sc["camera"], "lights": sc["lights"], "width": 48, "height": 40, "spp": 6}
+    for mc, flags in ((1, TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS), (2, 0)):
+        a, sa = _single(tree, sc, job, 3, mc, flags, 5, chunk=4096)
+        b, sb = _single(tree, sc, job, 3, mc, flags | TN_RENDER_PERSISTENT, 5, chunk=4096)
+        assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"]), mc
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6, err_msg=f"m {mc}")
+    scs = _scenes()
+    trees = [_tree(x) for x in scs]
+    jobs = _jobs(scs)
+    bt = Batch(trees)
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    ia, sa, _ = bt.render(jobs, max_depth=2, mc_samples=1, flags=flags, seed=9, chunk_paths=2048)
+    ib, sb, _ = bt.render(jobs, max_depth=2, mc_samples=1, flags=flags | TN_RENDER_PERSISTENT, seed=9,
+                          chunk_paths=2048)
+    assert (sa["rays"], sa["shadow_rays"]) == (sb["rays"], sb["shadow_rays"])
+    for s, (x, y) in enumerate(zip(ia, ib)):
+        np.testing.assert_allclose(y.cpu().numpy(), x.cpu().numpy(), rtol=1e-5, atol=1e-6, err_msg=f"scene {s}")
+    bt.close()
 
 
 def test_graph_replay_matches_direct(gpu):

@@ -37,6 +37,7 @@ TN_RENDER_GUIDED = 8
 TN_RENDER_EXHAUSTIVE = 16
 TN_RENDER_GRAPH = 32
 TN_RENDER_AUTOTUNE = 64
+TN_RENDER_PERSISTENT = 128
 TN_RENDERER_PATHTRACER = 0
 TN_RENDERER_RAYCASTER = 1
 TN_RENDERER_RAYTRACER = 2

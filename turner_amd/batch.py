@@ -9,9 +9,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import TN_RENDER_AUTOTUNE, TN_RENDER_GRAPH, check, lib, ptr
+from ._lib import TN_RENDER_AUTOTUNE, TN_RENDER_GRAPH, TN_RENDER_PERSISTENT, check, lib, ptr
 
-__all__ = ["Batch", "heterogeneous_workload", "TN_RENDER_AUTOTUNE", "TN_RENDER_GRAPH"]
+__all__ = ["Batch", "heterogeneous_workload", "TN_RENDER_AUTOTUNE", "TN_RENDER_GRAPH", "TN_RENDER_PERSISTENT"]
 
 
 def heterogeneous_workload(seed=2024, count=32):

@@ -97,6 +97,7 @@ struct Queues {
     // words apart (one per 64-byte line)
     uint32_t* xwork;
     unsigned long long* totals;  // [0] segments, [1] shadow rays
+    uint32_t* bar;  // grid barrier words of the persistent kernel (kBarWords)
 };
 
 // Per-scene parameters of a render: one entry per scene of a batch (one for
@@ -273,9 +274,7 @@ __global__ void k_chunk_totals(const uint32_t* cnt, unsigned long long* totals, 
 
 // Primary samples: within a scene, j = (y * width + x) * spp + i
 // (main.cpp:193-215 order); scenes follow each other in the sample space.
-__global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.cs->count) return;
+__device__ __forceinline__ void raygen_one(const RenderArgs& A, const Queues& Q, uint32_t i) {
     const unsigned long long j = A.cs->first + i;
     const uint32_t si = scene_of_sample(A, j);
     const SceneParams& S = A.sp[si];
@@ -307,6 +306,12 @@ __global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q) {
     Q.d[0][i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));  // .w: path within the chunk
     Q.w[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
     Q.rng[0][i] = rng;
+}
+
+__global__ __launch_bounds__(kRBlock) void k_raygen(RenderArgs A, Queues Q) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.cs->count) return;
+    raygen_one(A, Q, i);
 }
 
 // Caller rays (tn_trace): ray j accumulates into accum[j]; seeds from (seed, j).
@@ -382,11 +387,8 @@ __device__ __forceinline__ void shadow_lit(const RenderArgs& A, const Queues& Q,
 // record kFixId, which k_fix_hits replaces before anything reads it, and go
 // on the level's fix list.
 template <bool EXH, bool REF>
-__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
-    RenderArgs A, Queues Q, int level, int buf) {
-    using WL = std::conditional_t<REF, CoopLdsTie, CoopLds>;
-    __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ WL wl[kRBlock / 64];
+__device__ __forceinline__ void trace_body(const RenderArgs& A, const Queues& Q, int level, int buf, uint2* stk,
+                                           std::conditional_t<REF, CoopLdsTie, CoopLds>* wl) {
     const uint32_t n = min(Q.cnt[level], Q.cap);  // overflow is reported, never read
     const float4* qo = Q.o[buf];
     const float4* qd = Q.d[buf];
@@ -414,6 +416,14 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
         },
         cnt);
     if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
+}
+
+template <bool EXH, bool REF>
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(
+    RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ uint2 stk[kRStack * kRBlock];
+    __shared__ std::conditional_t<REF, CoopLdsTie, CoopLds> wl[kRBlock / 64];
+    trace_body<EXH, REF>(A, Q, level, buf, stk, wl);
 }
 
 // The segments of one level that k_trace<..., true> marked, traced one per
@@ -447,10 +457,8 @@ __global__ __launch_bounds__(kRBlock) void k_fix_hits(RenderArgs A, Queues Q, in
 // direction component is marked (sign bit of its tmax) and listed for
 // k_fix_shadows.
 template <bool EXH, bool REF>
-__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
-    RenderArgs A, Queues Q, int level) {
-    __shared__ uint2 stk[kRStack * kRBlock];
-    __shared__ CoopLds wl[kRBlock / 64];
+__device__ __forceinline__ void shadow_body(const RenderArgs& A, const Queues& Q, int level, uint2* stk,
+                                            CoopLds* wl) {
     const uint32_t n = min(Q.cnt[kMaxLevels + level], Q.scap);
     const int sb = level & 1;
     TraversalCounters cnt;
@@ -476,6 +484,14 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
         },
         cnt);
     if (cnt.incomplete) atomicAdd(A.status + 2, 1u);
+}
+
+template <bool EXH, bool REF>
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(
+    RenderArgs A, Queues Q, int level) {
+    __shared__ uint2 stk[kRStack * kRBlock];
+    __shared__ CoopLds wl[kRBlock / 64];
+    shadow_body<EXH, REF>(A, Q, level, stk, wl);
 }
 
 // The shadow rays of one level that k_shadow<..., true> marked, tested one
@@ -716,8 +732,8 @@ __global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) vo
 // Shade one level.  GUIDED kernels run one wave per block (the guide's
 // forward pass is separated by block barriers).
 template <bool GUIDED>
-__global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int level, int buf) {
-    __shared__ __align__(16) uint16_t glds[GUIDED ? 64 * (kGStrideIn + kGStrideHid) : 1];
+__device__ __forceinline__ void shade_body(const RenderArgs& A, const Queues& Q, int level, int buf,
+                                           uint16_t* glds) {
     const uint32_t n = min(Q.cnt[level], Q.cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -991,6 +1007,133 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
     }
 }
 
+template <bool GUIDED>
+__global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ __align__(16) uint16_t glds[GUIDED ? 64 * (kGStrideIn + kGStrideHid) : 1];
+    shade_body<GUIDED>(A, Q, level, buf, glds);
+}
+
+// ---------------------------------------------------------- persistent
+// Grid barrier of the persistent kernel (all blocks resident: cooperative
+// launch), two-level so that no counter sees more than a few hundred
+// atomics: the blocks count in at one of eight group counters (block b in
+// group b % 8, the XCD it is dealt to), the last of a group at the group
+// total, and the last group publishes the next generation, which the others
+// poll.  Words, each on its own 128-byte line: [0] generation, [32] set when
+// a wait ran out, [64] groups arrived, [96 + 32 g] arrivals in group g.  A
+// wait is bounded (2^22 polls of about 64 cycles): then [32] is set, and
+// every block stops at its next barrier (grid_sync returns false).
+constexpr uint32_t kBarrierPolls = 1u << 22;
+constexpr uint32_t kBarGen = 0, kBarOut = 32, kBarGroups = 64, kBarGroup = 96;
+constexpr uint32_t kBarWords = kBarGroup + 32 * kXcds;
+
+__device__ __forceinline__ bool grid_sync(uint32_t* bar, uint32_t& gen) {
+    __shared__ uint32_t ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t good = __hip_atomic_load(bar + kBarOut, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        if (good) {
+            __threadfence();
+            const uint32_t target = gen + 1u;
+            const uint32_t groups = gridDim.x < kXcds ? gridDim.x : kXcds;
+            const uint32_t g = blockIdx.x % groups;
+            const uint32_t members = gridDim.x / groups + (g < gridDim.x % groups ? 1u : 0u);
+            bool released = false;
+            if (atomicAdd(bar + kBarGroup + 32 * g, 1u) == members - 1u) {
+                atomicExch(bar + kBarGroup + 32 * g, 0u);  // reset before the group counts in
+                __threadfence();
+                if (atomicAdd(bar + kBarGroups, 1u) == groups - 1u) {
+                    atomicExch(bar + kBarGroups, 0u);
+                    __threadfence();
+                    atomicExch(bar + kBarGen, target);
+                    released = true;
+                }
+            }
+            if (!released) {
+                for (uint32_t polls = 0;
+                     __hip_atomic_load(bar + kBarGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target; ++polls) {
+                    if (polls >= kBarrierPolls) {
+                        atomicExch(bar + kBarOut, 1u);
+                        good = 0u;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __threadfence();
+        }
+        ok = good;
+    }
+    __syncthreads();
+    ++gen;
+    return ok != 0u;
+}
+
+// k_chunk_begin for one block of any size (block 0 of the persistent kernel).
+__device__ __forceinline__ void chunk_begin_block(uint32_t* cnt, uint32_t* xwork, ChunkState* cs) {
+    const unsigned long long first = cs->next;
+    const unsigned long long left = cs->total > first ? cs->total - first : 0ull;
+    const uint32_t count = (uint32_t)(left < cs->chunk ? left : cs->chunk);
+    for (uint32_t i = threadIdx.x; i < 4 * kMaxLevels + 2; i += blockDim.x)
+        if (i != 4 * kMaxLevels) cnt[i] = (i == 0) ? count : 0u;  // the overflow word lives for the whole render
+    for (uint32_t k = kFixHits + threadIdx.x; k < 6 * kMaxLevels + 2; k += blockDim.x) cnt[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < 2 * kMaxLevels * kXcds; k += blockDim.x) xwork[16 * k] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cs->first = first;
+        cs->count = count;
+        cs->next = first + cs->stride;
+    }
+}
+
+// Plain path tracing (no guide, no training records, no refined tree) of
+// every chunk of a call in one cooperative launch: per chunk, raygen, then
+// per level the same trace, shade and shadow stages as the wavefront
+// kernels, separated by grid barriers instead of kernel boundaries.  One
+// pipeline; the shadow test of a level no longer overlaps the next level's
+// tracing.  The chunk loop ends when a chunk is empty; every barrier wait is
+// bounded.
+template <bool EXH>
+__global__ __launch_bounds__(kRBlock) void k_persist(RenderArgs A, Queues Q, int levels) {
+    __shared__ uint2 stk[kRStack * kRBlock];
+    __shared__ CoopLds wl[kRBlock / 64];
+    __shared__ uint16_t glds[1];
+    uint32_t gen = 0;
+    for (;;) {
+        if (blockIdx.x == 0) chunk_begin_block(Q.cnt, Q.xwork, A.cs);
+        if (!grid_sync(Q.bar, gen)) return;
+        const uint32_t count = __hip_atomic_load(&A.cs->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (count == 0u) return;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+            raygen_one(A, Q, i);
+        if (!grid_sync(Q.bar, gen)) return;
+        int buf = 0;
+        for (int lv = 0; lv < levels; ++lv) {
+            trace_body<EXH, false>(A, Q, lv, buf, stk, wl);
+            if (!grid_sync(Q.bar, gen)) return;
+            shade_body<false>(A, Q, lv, buf, glds);
+            if (!grid_sync(Q.bar, gen)) return;
+            shadow_body<EXH, false>(A, Q, lv, stk, wl);
+            if (!grid_sync(Q.bar, gen)) return;
+            buf ^= 1;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {  // k_chunk_totals
+            unsigned long long a = 0, b = 0;
+            for (int k = 0; k < levels; ++k) {
+                a += Q.cnt[k];
+                b += Q.cnt[kMaxLevels + k];
+            }
+            Q.totals[0] += a;
+            Q.totals[1] += b;
+        }
+        __syncthreads();  // the counts are read before block 0 clears them for the next chunk
+    }
+}
+
+__global__ void k_zero_bar(uint32_t* bar) {
+    for (uint32_t i = threadIdx.x; i < kBarWords; i += blockDim.x) bar[i] = 0u;
+}
+
 // Clears the training records of a chunk: valid flags and found radiance.
 __global__ void k_rec_clear(RenderArgs A) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1130,6 +1273,7 @@ size_t carve(Queues& q, ChunkState** cs, char* base, uint32_t cap) {
     q.fix = (uint32_t*)take(3 * (size_t)kFixCap * sizeof(uint32_t));
     q.xwork = (uint32_t*)take(2 * kMaxLevels * kXcds * 16 * sizeof(uint32_t));
     q.totals = (unsigned long long*)take(2 * sizeof(unsigned long long));
+    q.bar = (uint32_t*)take(kBarWords * sizeof(uint32_t));
     *cs = (ChunkState*)take(sizeof(ChunkState));
     q.cap = cap;
     return off;
@@ -1279,6 +1423,7 @@ struct LaunchShape {
     uint32_t chunk = 0;
     uint32_t blocks_per_cu = 8;
     bool graph = false;
+    bool persistent = false;  // one cooperative launch walks every chunk and level (k_persist)
 };
 
 int check_render_params(const tn_render_params* P) {
@@ -1341,7 +1486,9 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     // pipeline: capturing the second pipeline's stream, which itself forks
     // its shadow stream, crashed the host process, so that combination
     // stays off.
-    int npipes = shape.graph ? 1 : kPipes;
+    // the persistent kernel serves plain path tracing of camera samples
+    const bool persistent = shape.persistent && renderer == TN_RENDERER_PATHTRACER && !guided && !rays && !G.refined;
+    int npipes = (shape.graph || persistent) ? 1 : kPipes;
     if (npipes > 1 && total <= chunk) {
         if (total >= (uint64_t)npipes << 19) {
             const uint64_t part = ((total + npipes - 1) / npipes + 255) & ~255ull;
@@ -1557,7 +1704,24 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     const uint64_t nchunks = (total + chunk - 1) / chunk;
     const uint64_t rounds = (nchunks + npipes - 1) / npipes;
     auto t0 = std::chrono::steady_clock::now();
-    if (shape.graph && rounds > 1) {
+    uint32_t barrier_out = 0;
+    if (persistent) {
+        // every block resident at once: the occupancy the kernel allows, at
+        // most the shape's blocks per CU
+        const void* kfn = exh ? (const void*)k_persist<true> : (const void*)k_persist<false>;
+        int per_cu = 0;
+        TN_RHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kRBlock, 0));
+        if (per_cu < 1) return fail(set_error(TN_ERR_LIMIT, "the persistent kernel does not fit a CU"));
+        const dim3 pgrid((uint32_t)G.cus * std::min<uint32_t>((uint32_t)per_cu, bpc));
+        k_zero_bar<<<1, 256, 0, stream>>>(Q[0].bar);
+        TN_RHIP(hipGetLastError());
+        RenderArgs pa = A[0];
+        Queues pq = Q[0];
+        int plevels = levels;
+        void* args[] = {&pa, &pq, &plevels};
+        TN_RHIP(hipLaunchCooperativeKernel(kfn, pgrid, dim3(kRBlock), args, 0, stream));
+        TN_RHIP(hipMemcpyAsync(&barrier_out, Q[0].bar + kBarOut, 4, hipMemcpyDeviceToHost, stream));
+    } else if (shape.graph && rounds > 1) {
         // The chunk sequence is captured on every call (host-side recording;
         // its kernel arguments change from call to call) and applied to the
         // executable graph kept from the last call by hipGraphExecUpdate,
@@ -1612,6 +1776,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (int p = 0; p < npipes; ++p)
         if (overflow[p]) return set_error(TN_ERR_QUEUE, "a ray queue overflowed");
+    if (barrier_out) return set_error(TN_ERR_INCOMPLETE, "a grid barrier wait of the persistent kernel ran out");
     if (incomplete) return set_error(TN_ERR_INCOMPLETE, "%u render traversals hit the visit cap", incomplete);
     if (st) {
         st->rays = st->shadow_rays = 0;
@@ -1647,6 +1812,7 @@ int render_into(tn_scene* sc, const tn_camera* cam, const tn_light* lights, uint
     if (!rays && (P->flags & TN_RENDER_REF_JITTER)) jit = reference_jitter(P->width, spp);
     LaunchShape shape;
     shape.graph = (P->flags & TN_RENDER_GRAPH) != 0;
+    shape.persistent = (P->flags & TN_RENDER_PERSISTENT) != 0;
     return render_core(geom_of(sc), table, total, P, shape, accum, st, stream, ray_o, ray_d,
                        jit.empty() ? nullptr : jit.data(), jit.size(), guide, train_samples);
 }
@@ -1771,7 +1937,9 @@ int autotune(tn_batch* b, const std::vector<SceneParams>& table, uint64_t total,
         if (chunk_size(P, c, ch, cap) != TN_OK || cap > (1u << 24)) continue;  // queue budget
         if (c > 2 * probe && c != chunks[0]) continue;
         for (uint32_t g : grids) {
-            // graph replay (one pipeline) or direct launches (two pipelines)
+            // graph replay (one pipeline) or direct launches (two
+            // pipelines); the persistent launch is no candidate (slower than
+            // both on every batch measured)
             for (const bool graph : {true, false}) {
                 LaunchShape sh;
                 sh.chunk = c;
@@ -2021,8 +2189,9 @@ int tn_batch_render(tn_batch* b, const tn_batch_job* jobs, const tn_render_param
     hipStream_t s = (hipStream_t)stream;
     LaunchShape shape;
     shape.graph = (P->flags & TN_RENDER_GRAPH) != 0;
+    shape.persistent = (P->flags & TN_RENDER_PERSISTENT) != 0;
     if ((P->flags & TN_RENDER_AUTOTUNE) && P->chunk_paths == 0) {
-        const uint32_t key_flags = P->flags & ~(TN_RENDER_GRAPH | TN_RENDER_AUTOTUNE);
+        const uint32_t key_flags = P->flags & ~(TN_RENDER_GRAPH | TN_RENDER_AUTOTUNE | TN_RENDER_PERSISTENT);
         int lg = 0;
         while ((1ull << lg) < total) ++lg;
         const tn_batch::Tuned* hit = nullptr;
