@@ -264,9 +264,12 @@ struct TrainSrc {
 };
 
 __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, float* __restrict__ partial) {
+    // 38 KB: four blocks per CU.  The outputs and the output deltas reuse
+    // sX (the features are computed again for the layer-1 gradient), and
+    // each layer's delta overwrites that layer's activations in place.
     __shared__ __align__(16) uint16_t sX[64 * kGStrideIn];
     __shared__ __align__(16) uint16_t sH[3][64 * kGStrideHid];
-    __shared__ __align__(16) uint16_t sD[2][64 * kGStrideHid];
+    static_assert(64 * kGOut * sizeof(float) <= sizeof(sX), "outputs fit the feature rows");
     const int l = threadIdx.x;
     const uint32_t i = blockIdx.x * 64u + (uint32_t)l;
     float3 p = make_float3(0.f, 0.f, 0.f), nn = p, w = p;
@@ -301,7 +304,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, flo
     __syncthreads();
     guide_hidden<4>(G.wbf + kGW3, G.theta + kGOffB3, sH[1], kGStrideHid, sH[2], kGStrideHid);
     __syncthreads();
-    float* sOut = reinterpret_cast<float*>(&sD[0][0]);
+    float* sOut = reinterpret_cast<float*>(sX);
     guide_output(G.wbf + kGW4, G.theta + kGOffB4, sH[2], kGStrideHid, sOut);
     __syncthreads();
     float o[kGOut], d[kGOut];
@@ -319,8 +322,8 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, flo
         cs += __shfl_xor(cs, off, 64);
         cl += __shfl_xor(cl, off, 64);
     }
-    __syncthreads();  // sOut (aliasing sD[0]) fully read
-    uint16_t* dO = sD[0];
+    __syncthreads();  // sOut (aliasing sX) fully read
+    uint16_t* dO = sX;
 #pragma unroll
     for (int k = 0; k < kGOut; ++k) dO[l * kGStrideHid + k] = to_bf16(d[k]);
 #pragma unroll
@@ -334,21 +337,24 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView G, TrainSrc S, flo
     // layer 4 (64 -> 40)
     weight_grad<3, 4>(dO, kGStrideHid, sH[2], kGStrideHid, P + kGOffW4, kGOut, kGHid);
     bias_grad(dO, kGStrideHid, kGOut, P + kGOffB4);
-    delta_back<3>(G.wbf + kGWT4, dO, sH[2], sD[1]);
+    // D3 replaces H3 (each element's mask is read by the lane that writes it)
+    delta_back<3>(G.wbf + kGWT4, dO, sH[2], sH[2]);
     __syncthreads();
     // layer 3
-    weight_grad<4, 4>(sD[1], kGStrideHid, sH[1], kGStrideHid, P + kGOffW3, kGHid, kGHid);
-    bias_grad(sD[1], kGStrideHid, kGHid, P + kGOffB3);
-    delta_back<4>(G.wbf + kGWT3, sD[1], sH[1], sD[0]);
+    weight_grad<4, 4>(sH[2], kGStrideHid, sH[1], kGStrideHid, P + kGOffW3, kGHid, kGHid);
+    bias_grad(sH[2], kGStrideHid, kGHid, P + kGOffB3);
+    delta_back<4>(G.wbf + kGWT3, sH[2], sH[1], sH[1]);
     __syncthreads();
     // layer 2
-    weight_grad<4, 4>(sD[0], kGStrideHid, sH[0], kGStrideHid, P + kGOffW2, kGHid, kGHid);
-    bias_grad(sD[0], kGStrideHid, kGHid, P + kGOffB2);
-    delta_back<4>(G.wbf + kGWT2, sD[0], sH[0], sD[1]);
+    weight_grad<4, 4>(sH[1], kGStrideHid, sH[0], kGStrideHid, P + kGOffW2, kGHid, kGHid);
+    bias_grad(sH[1], kGStrideHid, kGHid, P + kGOffB2);
+    delta_back<4>(G.wbf + kGWT2, sH[1], sH[0], sH[0]);
+    // the features again (dO is no longer read)
+    guide_features(G, p, nn, true, sX + l * kGStrideIn);
     __syncthreads();
     // layer 1 (78 -> 64)
-    weight_grad<4, 5>(sD[1], kGStrideHid, sX, kGStrideIn, P + kGOffW1, kGHid, kGIn);
-    bias_grad(sD[1], kGStrideHid, kGHid, P + kGOffB1);
+    weight_grad<4, 5>(sH[0], kGStrideHid, sX, kGStrideIn, P + kGOffW1, kGHid, kGIn);
+    bias_grad(sH[0], kGStrideHid, kGHid, P + kGOffB1);
 }
 
 // Two-stage deterministic reduction of the per-block gradient slices:

@@ -111,12 +111,28 @@ __device__ __forceinline__ float3 mixture_sample(const Mixture& M, float u0, flo
 typedef __attribute__((ext_vector_type(4))) float g_f4;
 typedef __attribute__((ext_vector_type(4))) short g_s4;
 
-// FP32 -> BF16, round to nearest even.
+// Two FP32 values -> packed BF16 (a low, b high), round to nearest even:
+// one v_cvt_pk_bf16_f32 on gfx950.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    typedef __attribute__((ext_vector_type(2))) float f2;
+    typedef __attribute__((ext_vector_type(2))) __bf16 h2;
+    const h2 h = __builtin_convertvector((f2){a, b}, h2);
+    uint32_t r;
+    __builtin_memcpy(&r, &h, 4);
+    return r;
+}
+
+// FP32 -> BF16, round to nearest even (finite x; the host form rounds the
+// bits, the device form converts in hardware, with equal results).
 __host__ __device__ __forceinline__ uint16_t to_bf16(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint16_t)pack_bf16x2(x, 0.f);
+#else
     uint32_t u;
     __builtin_memcpy(&u, &x, 4);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -132,15 +148,21 @@ __device__ __forceinline__ void guide_features(const GuideView& G, float3 p, flo
     row[0] = to_bf16(u[0]);
     row[1] = to_bf16(u[1]);
     row[2] = to_bf16(u[2]);
+    static_assert(kGFreq % 2 == 0, "frequencies come in pairs");
     int k = 3;
     for (int c = 0; c < 3; ++c) {
         float f = 1.0f;
-        for (int q = 0; q < kGFreq; ++q) {
+        for (int q = 0; q < kGFreq; q += 2) {
             float sv, cv;
             sincospif(f * u[c], &sv, &cv);  // sin / cos of 2^q pi u
+            // 2^(q+1) pi u by the double-angle identities: a few FP32 ulps
+            // from sincospif's values, far below the BF16 step
+            const float s2 = 2.0f * sv * cv, c2 = (cv - sv) * (cv + sv);
             row[k++] = to_bf16(sv);
             row[k++] = to_bf16(cv);
-            f *= 2.0f;
+            row[k++] = to_bf16(s2);
+            row[k++] = to_bf16(c2);
+            f *= 4.0f;
         }
     }
     row[75] = to_bf16(active ? n.x : 0.f);
@@ -307,9 +329,6 @@ __device__ __forceinline__ void mfma_wx32_mem(const uint16_t* __restrict__ W, co
     }
 }
 
-__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
-}
 
 // Epilogue of a hidden layer: bias, ReLU, BF16, one 8-byte LDS store of the
 // four consecutive features.
