@@ -84,6 +84,10 @@ def main():
     ap.add_argument("--tree-probe", action="store_true",
                     help="reference vs rendering kd-tree on render-like ray batches of the "
                          "occluded-light scene (camera, shadow, bounce), with work counters, then exit")
+    ap.add_argument("--launch-probe", action="store_true",
+                    help="the 32-scene batch and a small 4-scene batch rendered with each launch form "
+                         "(direct launches on two pipelines, graph replay, one persistent cooperative "
+                         "launch) at fixed chunk sizes, then exit")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one traversal launch plus a 1 GiB device copy (the calibration "
                          "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE), then exit")
@@ -105,6 +109,9 @@ def main():
         return
     if args.tree_probe:
         print(json.dumps(tree_probe()), flush=True)
+        return
+    if args.launch_probe:
+        print(json.dumps(launch_probe()), flush=True)
         return
     dev = torch.device("cuda:0")
     name, cus = T.device_info()
@@ -360,6 +367,62 @@ def batch_throughput(args):
                     "expected": 0.8, "sphere_pixels": int(on.sum()), "spp": jobs[0]["spp"],
                     "enclosure_max_abs_err": round(float(np.abs(f[off][:, :3] - 1.0).max()), 7)},
     }
+
+
+def launch_probe():
+    """Launch forms of a batch render at fixed chunk sizes: direct launches
+    (two pipelines, shadow tests on side streams), graph replay of the
+    per-chunk sequence (one pipeline) and TN_RENDER_PERSISTENT (one
+    cooperative launch walking every chunk and level behind grid barriers).
+    Best wall time of a few renders after one untimed render per form; the
+    segment and shadow-ray counts of the forms must agree."""
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import (TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE, TN_RENDER_GRAPH,
+                                 TN_RENDER_PERSISTENT)
+    from turner_amd.batch import Batch, heterogeneous_workload
+    base = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    forms = (("direct", 0), ("graph", TN_RENDER_GRAPH), ("persistent", TN_RENDER_PERSISTENT))
+
+    def timed(b, jobs, chunk, reps):
+        out, counts = {}, set()
+        for name, f in forms:
+            b.render(jobs, max_depth=3, mc_samples=1, flags=base | f, seed=1, chunk_paths=chunk)
+            best = float("inf")
+            for r in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, st, _ = b.render(jobs, max_depth=3, mc_samples=1, flags=base | f, seed=2 + r, chunk_paths=chunk)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+                if r == 0:
+                    counts.add((st["rays"], st["shadow_rays"]))
+            out[name] = {"ms": round(best * 1e3, 2), "closest_hit_mrays_s": round(st["rays"] / best / 1e6, 1)}
+        assert len(counts) == 1, counts  # same paths whatever the launch form
+        return out
+
+    res = {}
+    scs, _, jobs = heterogeneous_workload(seed=2024)
+    trees = [T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True) for sc in scs]
+    b = Batch(trees)
+    note("launch probe: 32-scene batch")
+    res["batch_32_scenes_chunk_2p22"] = timed(b, jobs, 1 << 22, 2)
+    b.close()
+    small = [scenes.cornell_box(), scenes.sphere_in_cornell(level=5), scenes.furnace(level=4),
+             scenes.occluded_light_cornell(level=5)]
+    trees = [T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True) for sc in small]
+    sjobs = [{"camera": sc["camera"], "lights": sc["lights"], "width": 128, "height": 128, "spp": 4} for sc in small]
+    b = Batch(trees)
+    for c in (14, 16, 18):
+        note(f"launch probe: 4 scenes of 128x128x4, chunk 2^{c}")
+        res[f"batch_4_small_scenes_chunk_2p{c}"] = timed(b, sjobs, 1 << c, 5)
+    b.close()
+    return {"workload": "max-depth 3, m 1, cosine + area-light NEE, rendering kd-trees; 32 scenes = the batch "
+                        "section's heterogeneous workload (seed 2024); 4 small scenes = Cornell box, sphere in "
+                        "Cornell (level 5), furnace (level 4), occluded light (level 5) at 128x128, 4 spp",
+            "forms": res}
 
 
 def tree_probe():
