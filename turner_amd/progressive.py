@@ -27,7 +27,7 @@ def _conf(width, height, depth, flags, spp, seed):
 def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=64,
                    train_passes=16, train_samples=1 << 18, train_spp=2, max_seconds=60.0, seed=1,
                    max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0,
-                   passes_per_render=1, min_pass=0):
+                   passes_per_render=1, min_pass=0, image_mis=None):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
     reference's per-pixel variance (relmse subtracts it).
@@ -43,7 +43,10 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
     min_pass=q > 0 sizes each image pass after the first from the error so
     far: relMSE falls as 1/spp, so about spp * err / target samples reach the
     target; the next pass takes the missing samples rounded up to a multiple
-    of q, at most spp_per_pass (the same rule for both tracers)."""
+    of q, at most spp_per_pass (the same rule for both tracers).
+    image_mis (guided): the guide's share of the one-sample MIS mixture from
+    the first image pass on (the training passes keep the guide's own
+    setting); every pass stays unbiased whatever the share."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -63,6 +66,8 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
         if not training and min_pass > 0 and spp_img > 0 and err > target and err != float("inf"):
             missing = spp_img * err / target - spp_img
             n = int(min(spp_per_pass, max(min_pass, -(-missing // min_pass) * min_pass)))
+        if guide is not None and image_mis is not None and k == train_renders:
+            guide.set_options(guide.lr, guide.batch, image_mis, guide.epochs)
         conf = _conf(w, h, depth, flags, n, seed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
