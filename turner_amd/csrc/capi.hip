@@ -196,10 +196,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     coop_loop<kStack, EXH, MT, ANY, COUNT, TIE>(
         sc, stk + threadIdx.x, kBlock, wl + threadIdx.x / 64,
         RayChunks{work, n, chunk, gridDim.x * (kBlock / 64), wave * chunk, blockIdx.x % kXcds, false},
+        // rays and results stream through once: non-temporal loads and
+        // stores keep them from evicting tree nodes and records from L2
         [&](uint32_t k, float3& o, float3& d, float& tm, uint32_t&) {
-            o = make_float3(o3[3 * (size_t)k], o3[3 * (size_t)k + 1], o3[3 * (size_t)k + 2]);
-            d = make_float3(d3[3 * (size_t)k], d3[3 * (size_t)k + 1], d3[3 * (size_t)k + 2]);
-            if (ANY) tm = tmax[k];
+            const float* po = o3 + 3 * (size_t)k;
+            const float* pd = d3 + 3 * (size_t)k;
+            o = make_float3(__builtin_nontemporal_load(po), __builtin_nontemporal_load(po + 1),
+                            __builtin_nontemporal_load(po + 2));
+            d = make_float3(__builtin_nontemporal_load(pd), __builtin_nontemporal_load(pd + 1),
+                            __builtin_nontemporal_load(pd + 2));
+            if (ANY) tm = __builtin_nontemporal_load(tmax + k);
             if (COUNT) nrays++;
         },
         [&](uint32_t k, const Trav& T, bool traversed) {
@@ -211,12 +217,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 return;
             }
             if (ANY) {
-                occ[k] = hit ? 1 : 0;
+                __builtin_nontemporal_store((uint8_t)(hit ? 1 : 0), occ + k);
             } else {
-                out_id[k] = hit ? T.best_id : kInvalidId;
-                out_r[k] = hit ? T.best_r : FLT_MAX;
-                out_s[k] = hit ? T.best_s : 0.f;
-                out_t[k] = hit ? T.best_t : 0.f;
+                __builtin_nontemporal_store(hit ? T.best_id : kInvalidId, out_id + k);
+                __builtin_nontemporal_store(hit ? T.best_r : FLT_MAX, out_r + k);
+                __builtin_nontemporal_store(hit ? T.best_s : 0.f, out_s + k);
+                __builtin_nontemporal_store(hit ? T.best_t : 0.f, out_t + k);
             }
         },
         cnt);
