@@ -800,9 +800,10 @@ def guided_vs_plain(args):
     plain = time_to_relmse(tree, sc, ref, depth, flags, seed=11, ref_var=ref_var, spp_per_pass=32,
                            min_pass=args.guided_min_pass)
     # guide options and training schedule (swept on this scene over five seed pairs: guide share 0.5..0.9,
-    # 4..10 training passes, 4..6 epochs)
+    # 4..10 training passes, 4..6 epochs, training in order or deferred beside the next pass's render)
     gopt = {"epochs": 5, "mis_guided": 0.7}
-    sched = {"train_passes": 6, "train_spp": 1, "train_samples": 1 << 18, "keep_training_passes": False}
+    sched = {"train_passes": 6, "train_spp": 1, "train_samples": 1 << 18, "keep_training_passes": False,
+             "defer_training": True}
     g = Guide.for_tree(tree, seed=2, **gopt)
     guided = time_to_relmse(tree, sc, ref, depth, flags, guide=g, seed=12, ref_var=ref_var, spp_per_pass=32,
                             min_pass=args.guided_min_pass, **sched)
@@ -856,8 +857,10 @@ def guided_vs_plain(args):
                      f"plain reference ({ref_s:.1f} s; its own per-pixel variance, relMSE "
                      f"{ref_noise:.4f}, is subtracted)"),
         "guided_schedule": ("6 training passes of 1 spp, each training on a uniform 2^18-sample subset of "
-                            "its radiance records for 5 epochs (Adam lr 1e-3, batch 16384), then image "
-                            "passes; training passes are timed but left out of the image; one-sample MIS "
+                            "its radiance records for 5 epochs (Adam lr 1e-3, batch 16384) beside the next "
+                            "training pass's render (deferred training; the last set trains before the image "
+                            "passes), then image passes; training passes are timed but left out of the "
+                            "image; one-sample MIS "
                             "0.7 guide / 0.3 cosine; both tracers add image passes of 32 spp, each after "
                             "the first sized from the error so far (relMSE ~ 1/spp: the missing samples, "
                             f"rounded up to a multiple of {args.guided_min_pass} spp, at most 32), and sample "

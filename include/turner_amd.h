@@ -252,6 +252,13 @@ typedef struct {
  * no shadow/trace overlap, the union of the stages' registers), so the
  * autotuner does not pick it.  Ignored where it does not apply. */
 #define TN_RENDER_PERSISTENT 128u
+/* tn_render_guided: the records of a training render are not trained at the
+ * end of its call but during the next tn_render_guided call with this flag
+ * (on the guide's own stream, beside that call's render, which samples from a
+ * copy of the weights taken before that training starts), or by
+ * tn_guide_flush / tn_guide_train_recorded.  Every call still returns with
+ * its work done.  No reference counterpart. */
+#define TN_RENDER_DEFER_TRAINING 256u
 
 typedef struct {
     uint32_t width, height;
@@ -381,6 +388,10 @@ int tn_guide_info(tn_guide* g, uint32_t* train_samples, uint32_t* steps, float* 
  * Adam steps (as tn_render_guided trains after rendering).  Lets one render
  * supply the data of several training passes.  Asynchronous on `stream`. */
 int tn_guide_train_recorded(tn_guide* g, uint32_t train_samples, void* stream);
+/* Trains the records a TN_RENDER_DEFER_TRAINING render left untrained (if
+ * any), on stream; afterwards nothing is pending.  Asynchronous on stream.
+ * No reference counterpart. */
+int tn_guide_flush(tn_guide* g, void* stream);
 /* Copies up to cap samples of that training set to host memory, 12 floats
  * each: p.xyz, c, n.xyz, cos/pdf, w.xyz, tag (uint32 bits: the path's sample
  * index within its render call, modulo 2^26, << 6 | bounce level).  Returns

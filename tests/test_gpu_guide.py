@@ -482,3 +482,73 @@ def test_train_recorded_takes_more_passes(gpu):
     assert after["steps"] == info["steps"] + 4 and after["train_samples"] == info["train_samples"]
     assert np.isfinite(after["loss"]) and not np.array_equal(g.params(), theta)
     g.close()
+
+
+def test_deferred_training(gpu):
+    """TN_RENDER_DEFER_TRAINING: a training render leaves its records pending;
+    the next deferred call trains them on the guide's stream beside its own
+    render, which samples from the weights as they were before that training
+    (its image equals a render by an untrained copy of the guide, up to the
+    FP32 accumulation order); tn_guide_flush trains what is left; a guide
+    trained this way lowers the error like one trained in order."""
+    torch = _torch()
+    from turner_amd.guide import Guide, render_guided
+    sc = scenes.sphere_in_cornell(level=3)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    flags = TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS
+    conf = T.TracerConfig(width=32, max_recursion_depth=3, num_pixel_samples=8, num_monte_carlo_samples=1,
+                          flags=flags, seed=5)
+    g = Guide.for_tree(tree, seed=9, batch=4096, epochs=2)
+    theta0 = g.params()
+    acc = torch.zeros((32, 32, 4), dtype=torch.float32, device="cuda")
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1 << 12, defer=True)
+    assert g.info()["steps"] == 0 and np.array_equal(g.params(), theta0)  # pending, not trained
+    # an image pass: the pending set trains beside it, the image uses theta0
+    img = torch.zeros((32, 32, 4), dtype=torch.float32, device="cuda")
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, img, sample_offset=8, defer=True)
+    assert g.info()["steps"] == 2 and not np.array_equal(g.params(), theta0)
+    g0 = Guide.for_tree(tree, seed=9, batch=4096, epochs=2)
+    assert np.array_equal(g0.params(), theta0)
+    ref = torch.zeros((32, 32, 4), dtype=torch.float32, device="cuda")
+    render_guided(tree, sc["camera"], sc["lights"], conf, g0, ref, sample_offset=8)
+    np.testing.assert_allclose(img.cpu().numpy(), ref.cpu().numpy(), rtol=1e-5, atol=1e-6)
+    g0.close()
+    # two training renders in a row: the first's set trains during the second
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1 << 12, defer=True)
+    render_guided(tree, sc["camera"], sc["lights"], conf, g, acc, train_samples=1 << 12, defer=True)
+    assert g.info()["steps"] == 4
+    g.flush()
+    torch.cuda.synchronize()
+    assert g.info()["steps"] == 6
+    g.flush()  # nothing pending
+    torch.cuda.synchronize()
+    assert g.info()["steps"] == 6 and np.isfinite(g.info()["loss"])
+    g.close()
+    # the guiding gain with deferred training (the occluded-light case)
+    from turner_amd.relmse import reference_image, relmse, render_plain
+    sc = scenes.occluded_light_cornell(level=5)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"])
+    w, depth = 64, 6
+    ref = reference_image(tree, sc, w, depth, flags, spp=4096, seed=101)
+    g = Guide.for_tree(tree, seed=11, batch=16384)
+    tconf = T.TracerConfig(width=w, max_recursion_depth=depth, num_pixel_samples=16, num_monte_carlo_samples=1,
+                           flags=flags, seed=12)
+    tacc = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
+    for k in range(30):
+        render_guided(tree, sc["camera"], sc["lights"], tconf, g, tacc, train_samples=1 << 16,
+                      sample_offset=16 * k, defer=True)
+    g.flush()
+    spp = 64
+    errs_g, errs_p = [], []
+    for rep in range(4):
+        c = T.TracerConfig(width=w, max_recursion_depth=depth, num_pixel_samples=spp, num_monte_carlo_samples=1,
+                           flags=flags, seed=200 + rep)
+        a = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
+        render_guided(tree, sc["camera"], sc["lights"], c, g, a, defer=True)
+        errs_g.append(relmse(a / spp, ref))
+        errs_p.append(relmse(render_plain(tree, sc, c) / spp, ref))
+    eg, ep = float(np.mean(errs_g)), float(np.mean(errs_p))
+    print(f"[deferred guiding gain] relMSE at {spp} spp: guided {eg:.5f}, cosine {ep:.5f} ({ep / eg:.2f}x)",
+          flush=True)
+    assert eg < 0.8 * ep
+    g.close()

@@ -38,6 +38,7 @@ TN_RENDER_EXHAUSTIVE = 16
 TN_RENDER_GRAPH = 32
 TN_RENDER_AUTOTUNE = 64
 TN_RENDER_PERSISTENT = 128
+TN_RENDER_DEFER_TRAINING = 256
 TN_RENDERER_PATHTRACER = 0
 TN_RENDERER_RAYCASTER = 1
 TN_RENDERER_RAYTRACER = 2
@@ -141,6 +142,7 @@ SIGNATURES = {
     "tn_guide_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), _FP]),
     "tn_guide_get_train_set": (C.c_long, [_P, _P, C.c_size_t]),
     "tn_guide_train_recorded": (C.c_int, [_P, _U32, _P]),
+    "tn_guide_flush": (C.c_int, [_P, _P]),
     "tn_render_guided": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(Light), _U32,
                                    C.POINTER(RenderParams), _P, _U32, _P,
                                    C.POINTER(RenderStats), _P]),

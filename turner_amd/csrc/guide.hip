@@ -16,8 +16,10 @@
 // with respect to the 40 outputs is evaluated in FP32 in log space.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <random>
 #include <vector>
 
@@ -515,6 +517,42 @@ int guide_reserve_set(tn_guide* g, uint32_t cap, hipStream_t stream) {
     return TN_OK;
 }
 
+int guide_train_pending(tn_guide* g, hipStream_t stream) {
+    if (!g->pending) return TN_OK;
+    const uint32_t n = g->pending;
+    g->pending = 0;
+    return guide_train_from_set(g, n, stream);
+}
+
+int guide_defer_begin(tn_guide* g, hipStream_t stream, bool recording, bool* started) {
+    *started = false;
+    if (!g->pending) return TN_OK;
+    if (!g->tstream) {
+        TN_HIP(hipStreamCreateWithFlags(&g->tstream, hipStreamNonBlocking));
+        TN_HIP(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+        TN_HIP(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
+    }
+    TN_HIP(hipMemcpyAsync(g->wbf_snap, g->wbf, kGBf16 * sizeof(uint16_t), hipMemcpyDeviceToDevice, stream));
+    TN_HIP(hipMemcpyAsync(g->theta_snap, g->theta, kGParams * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    TN_HIP(hipEventRecord(g->ev_fork, stream));
+    TN_HIP(hipStreamWaitEvent(g->tstream, g->ev_fork, 0));
+    // the kernels take the set's pointers by value, so the swap below
+    // leaves this training on the pending set
+    const int rc = guide_train_pending(g, g->tstream);
+    if (rc) {
+        (void)hipStreamSynchronize(g->tstream);
+        return rc;
+    }
+    TN_HIP(hipEventRecord(g->ev_join, g->tstream));
+    if (recording) {
+        std::swap(g->ts, g->ts_alt);
+        std::swap(g->ts_cap, g->ts_alt_cap);
+        std::swap(g->ts_count, g->ts_count_alt);
+    }
+    *started = true;
+    return TN_OK;
+}
+
 int guide_train_from_set(tn_guide* g, uint32_t train_samples, hipStream_t stream) {
     const uint32_t nb = g->batch / 64;
     int rc = ensure_partial(g, nb);
@@ -585,8 +623,11 @@ int tn_guide_create(const float bmin[3], const float bmax[3], uint64_t seed, tn_
         (e = hipMalloc(&g->adam_v, kGParams * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&g->wbf, kGBf16 * sizeof(uint16_t))) != hipSuccess ||
         (e = hipMalloc(&g->dloss, 4 * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&g->ts_count, 4 * sizeof(uint32_t))) != hipSuccess)
+        (e = hipMalloc(&g->ts_count, 4 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&g->wbf_snap, kGBf16 * sizeof(uint16_t))) != hipSuccess ||
+        (e = hipMalloc(&g->theta_snap, kGParams * sizeof(float))) != hipSuccess)
         return fail(hip_error(e, "hipMalloc"));
+    g->ts_count_alt = g->ts_count + 2;  // words 0 and 2 count the two sets
     if ((e = hipMemset(g->dloss, 0, 4 * sizeof(float))) != hipSuccess ||
         (e = hipMemset(g->ts_count, 0, 4 * sizeof(uint32_t))) != hipSuccess)
         return fail(hip_error(e, "hipMemset"));
@@ -600,11 +641,23 @@ int tn_guide_destroy(tn_guide* g) {
     if (!g) return TN_OK;
     (void)hipSetDevice(g->device);
     (void)hipDeviceSynchronize();
+    // ts_count and ts_count_alt are two words of one allocation
+    uint32_t* counts = std::min(g->ts_count, g->ts_count_alt ? g->ts_count_alt : g->ts_count);
     for (void* p : {(void*)g->theta, (void*)g->adam_m, (void*)g->adam_v, (void*)g->wbf, (void*)g->partial,
-                    (void*)g->dloss, (void*)g->ts, (void*)g->ts_count})
+                    (void*)g->dloss, (void*)g->ts, (void*)g->ts_alt, (void*)counts, (void*)g->wbf_snap,
+                    (void*)g->theta_snap})
         if (p) (void)hipFree(p);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    if (g->tstream) (void)hipStreamDestroy(g->tstream);
     delete g;
     return TN_OK;
+}
+
+int tn_guide_flush(tn_guide* g, void* stream) {
+    int rc = check_guide(g);
+    if (rc) return rc;
+    return guide_train_pending(g, (hipStream_t)stream);
 }
 
 int tn_guide_get_params(tn_guide* g, float* theta) {
@@ -718,6 +771,7 @@ int tn_guide_train_recorded(tn_guide* g, uint32_t train_samples, void* stream) {
     if (rc) return rc;
     if (train_samples == 0) return set_error(TN_ERR_INVALID, "train_samples must be positive");
     if (!g->ts) return set_error(TN_ERR_INVALID, "no recorded training set: render with train_samples > 0 first");
+    if ((rc = guide_train_pending(g, (hipStream_t)stream))) return rc;  // a deferred set trains first
     return guide_train_from_set(g, train_samples, (hipStream_t)stream);
 }
 

@@ -1509,7 +1509,9 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     // call used, so that no later call (once the lock is released) reuses
     // queues or records that kernels of this one may still read.
     bool enqueued = false;
+    bool deferred = false;  // a deferred training set trains on the guide's stream during this call
     auto fail = [&](int code) {
+        if (deferred) (void)hipStreamSynchronize(guide->tstream);
         if (enqueued) {
             (void)hipStreamSynchronize(stream);
             for (int p = 0; p < kPipes; ++p) {
@@ -1571,8 +1573,20 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
     A0.seed = P->seed;
     A0.accum = (float4*)accum;
     A0.seed_pass = P->seed + (uint64_t)P->sample_offset * 0xD1B54A32D192ED03ull;
+    // deferred training: the records of the last render train on the
+    // guide's stream while this one renders from a snapshot of the weights
+    const bool defer = guided && (P->flags & TN_RENDER_DEFER_TRAINING);
+    if (guided && guide->pending) {
+        if (defer) rc = guide_defer_begin(guide, stream, training, &deferred);
+        else rc = guide_train_pending(guide, stream);
+        if (rc) return fail(rc);
+    }
     if (guided) {
         A0.gv = guide->view();
+        if (deferred) {
+            A0.gv.wbf = guide->wbf_snap;
+            A0.gv.theta = guide->theta_snap;
+        }
         A0.mis = guide->mis_guided;
     }
     if (training) {
@@ -1762,7 +1776,11 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
         }
     }
     TN_RHIP(hipGetLastError());
-    if (training && (rc = guide_train_from_set(guide, train_samples, stream))) return fail(rc);
+    if (training) {
+        if (defer) guide->pending = train_samples;  // trains during the next deferred call (or tn_guide_flush)
+        else if ((rc = guide_train_from_set(guide, train_samples, stream))) return fail(rc);
+    }
+    if (deferred) TN_RHIP(hipStreamWaitEvent(stream, guide->ev_join, 0));
     unsigned long long tot[kPipes][2] = {};
     uint32_t overflow[kPipes] = {}, incomplete = 0;
     for (int p = 0; p < npipes; ++p) {

@@ -443,6 +443,18 @@ struct tn_guide {
     float4* ts = nullptr;     // training set: 3 float4 per sample (p, c), (n, cos/pdf), (w, tag)
     uint32_t ts_cap = 0;
     uint32_t* ts_count = nullptr;
+    // Deferred training (TN_RENDER_DEFER_TRAINING): the records of the last
+    // render wait in ts (pending = its train_samples) and train during the
+    // next render, on tstream, while that render samples from a snapshot of
+    // the weights and records into the other set (ts_alt; the two swap).
+    float4* ts_alt = nullptr;
+    uint32_t ts_alt_cap = 0;
+    uint32_t* ts_count_alt = nullptr;
+    uint32_t pending = 0;
+    uint16_t* wbf_snap = nullptr;
+    float* theta_snap = nullptr;
+    hipStream_t tstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
     uint64_t seed = 1;
     uint32_t steps = 0;
@@ -472,4 +484,11 @@ int guide_reserve_set(tn_guide* g, uint32_t cap, hipStream_t stream);
 // train_samples / batch Adam steps on mini-batches drawn uniformly (with
 // replacement) from the training set.
 int guide_train_from_set(tn_guide* g, uint32_t train_samples, hipStream_t stream);
+// Trains the pending (deferred) set now, on stream.
+int guide_train_pending(tn_guide* g, hipStream_t stream);
+// Starts the pending set's training on the guide's stream after stream's
+// current point, with a snapshot of the weights taken first on stream for
+// the render to sample from; recording: the render's records go to the
+// other set.  *started is false when nothing was pending.
+int guide_defer_begin(tn_guide* g, hipStream_t stream, bool recording, bool* started);
 }  // namespace tn

@@ -126,6 +126,12 @@ class Guide:
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
         check(lib().tn_guide_train_recorded(self.handle, train_samples, s))
 
+    def flush(self, stream=None):
+        """Trains the records a deferred training render left (tn_guide_flush)."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        check(lib().tn_guide_flush(self.handle, s))
+
     def info(self):
         ns, st, loss = C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().tn_guide_info(self.handle, C.byref(ns), C.byref(st), C.byref(loss)))
@@ -145,11 +151,13 @@ class Guide:
                 "w": buf[:, 8:11], "path": tag >> 6, "level": tag & 63}
 
 
-def render_guided(tree, cam, lights, conf, guide, accum, train_samples=0, sample_offset=0):
+def render_guided(tree, cam, lights, conf, guide, accum, train_samples=0, sample_offset=0, defer=False):
     """One guided render call adding per-pixel radiance sums into accum (a
     (h, w, 4) float32 device tensor).  train_samples > 0 (needs
     num_monte_carlo_samples == 1) also records radiance samples and trains
-    the guide on train_samples of them afterwards.  Returns the stats dict."""
+    the guide on train_samples of them afterwards -- or, with defer=True
+    (TN_RENDER_DEFER_TRAINING), during the next deferred call, beside its
+    render (guide.flush() trains them at once).  Returns the stats dict."""
     from . import _camera, _lights, _params
     w, h = conf.width, conf.height()
     accum = _dev(accum)
@@ -157,6 +165,8 @@ def render_guided(tree, cam, lights, conf, guide, accum, train_samples=0, sample
         raise ValueError("accum must hold width * height * 4 floats")
     p = _params(conf, w, h)
     p.sample_offset = sample_offset
+    if defer:
+        p.flags |= _lib.TN_RENDER_DEFER_TRAINING
     st = _lib.RenderStats()
     check(lib().tn_render_guided(tree.handle, C.byref(_camera(cam)), _lights(lights), len(lights),
                                  C.byref(p), guide.handle, train_samples, ptr(accum), C.byref(st),

@@ -27,7 +27,7 @@ def _conf(width, height, depth, flags, spp, seed):
 def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_per_pass=64,
                    train_passes=16, train_samples=1 << 18, train_spp=2, max_seconds=60.0, seed=1,
                    max_spp=1 << 16, ref_var=None, keep_training_passes=False, train_every=0,
-                   passes_per_render=1, min_pass=0, image_mis=None):
+                   passes_per_render=1, min_pass=0, image_mis=None, defer_training=False):
     """Returns a dict: seconds (timed render + training), spp, relmse,
     reached (bool), rays, history [(seconds, spp, relmse)].  ref_var: the
     reference's per-pixel variance (relmse subtracts it).
@@ -46,7 +46,12 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
     of q, at most spp_per_pass (the same rule for both tracers).
     image_mis (guided): the guide's share of the one-sample MIS mixture from
     the first image pass on (the training passes keep the guide's own
-    setting); every pass stays unbiased whatever the share."""
+    setting); every pass stays unbiased whatever the share.
+    defer_training (guided): each training pass's records train during the
+    next training pass's render (TN_RENDER_DEFER_TRAINING); the last set
+    trains before the first image pass (tn_guide_flush), so every image pass
+    samples from the final weights.  Every call returns with its training
+    done, so the timed windows hold all of it."""
     from .guide import render_guided
     h, w = ref.shape[0], ref.shape[1]
     acc = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -71,11 +76,14 @@ def time_to_relmse(tree, scene, ref, depth, flags, target=0.01, guide=None, spp_
         conf = _conf(w, h, depth, flags, n, seed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        if guide is not None and defer_training and k == train_renders:
+            guide.flush()  # the last training pass's records, before the image passes
         if guide is None:
             st = render_accumulate(tree, scene["camera"], scene["lights"], conf, acc, sample_offset=spp)
         else:
             st = render_guided(tree, scene["camera"], scene["lights"], conf, guide, tacc if training else acc,
-                               train_samples=train_samples if (training or late) else 0, sample_offset=spp)
+                               train_samples=train_samples if (training or late) else 0, sample_offset=spp,
+                               defer=defer_training and training)
             for _ in range(extra):
                 guide.train_recorded(train_samples)
         torch.cuda.synchronize()
