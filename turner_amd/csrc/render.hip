@@ -1013,6 +1013,16 @@ __global__ __launch_bounds__(kRBlock) void k_shade(RenderArgs A, Queues Q, int l
     shade_body<GUIDED>(A, Q, level, buf, glds);
 }
 
+// The unguided shade capped at 72 VGPRs (7 waves per SIMD, 68 bytes of
+// spill) instead of its natural 99 (5 waves): at 1920x1080, depth 3, m 4
+// the render went 1151 -> 1191 Mrays/s on the rendering tree (6 waves, 80
+// VGPRs: 1157).
+__global__ __launch_bounds__(kRBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_shade_plain(
+    RenderArgs A, Queues Q, int level, int buf) {
+    __shared__ uint16_t glds[1];
+    shade_body<false>(A, Q, level, buf, glds);
+}
+
 // ---------------------------------------------------------- persistent
 // Grid barrier of the persistent kernel (all blocks resident: cooperative
 // launch), two-level so that no counter sees more than a few hundred
@@ -1670,7 +1680,7 @@ int render_core(const RenderGeom& G, const std::vector<SceneParams>& table, uint
             if (dep >= 0) chk(hipStreamWaitEvent(s, rb.ev_shadow[dep], 0));
             if (renderer == TN_RENDERER_RAYTRACER) k_rt_shade<<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
             else if (guided) k_shade<true><<<tgrid.x * (kRBlock / 64), 64, 0, s>>>(Ap, Qp, lv, buf);
-            else k_shade<false><<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
+            else k_shade_plain<<<tgrid, tblock, 0, s>>>(Ap, Qp, lv, buf);
             chk(hipEventRecord(rb.ev_shade[lv], s));
             chk(hipStreamWaitEvent(side, rb.ev_shade[lv], 0));
             if (renderer == TN_RENDERER_RAYTRACER) {
