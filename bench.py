@@ -88,6 +88,9 @@ def main():
                     help="the 32-scene batch and a small 4-scene batch rendered with each launch form "
                          "(direct launches on two pipelines, graph replay, one persistent cooperative "
                          "launch) at fixed chunk sizes, then exit")
+    ap.add_argument("--guide-pass-probe", action="store_true",
+                    help="1-spp guided training passes of the occluded-light scene (512x512, depth 6) timed "
+                         "with the records trained in order and deferred beside the next render, then exit")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one traversal launch plus a 1 GiB device copy (the calibration "
                          "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE), then exit")
@@ -112,6 +115,9 @@ def main():
         return
     if args.launch_probe:
         print(json.dumps(launch_probe()), flush=True)
+        return
+    if args.guide_pass_probe:
+        print(json.dumps(guide_pass_probe()), flush=True)
         return
     dev = torch.device("cuda:0")
     name, cus = T.device_info()
@@ -367,6 +373,46 @@ def batch_throughput(args):
                     "expected": 0.8, "sphere_pixels": int(on.sum()), "spp": jobs[0]["spp"],
                     "enclosure_max_abs_err": round(float(np.abs(f[off][:, :3] - 1.0).max()), 7)},
     }
+
+
+def guide_pass_probe():
+    """Six 1-spp training passes (2^18 records each, 5 epochs of 16,384-sample
+    Adam steps) and one image pass of the guided tracer on the occluded-light
+    scene, 512x512, depth 6, rendering kd-tree: wall time per call (torch-
+    synchronised) with the records trained at the end of each call and with
+    TN_RENDER_DEFER_TRAINING (trained on the guide's stream beside the next
+    call's render).  Two rounds each; the first call of a round allocates."""
+    import torch
+
+    import turner_amd as T
+    from turner_amd import scenes
+    from turner_amd._lib import TN_RENDER_AREA_LIGHTS, TN_RENDER_COSINE
+    from turner_amd.guide import Guide, render_guided
+    sc = scenes.occluded_light_cornell(level=8)
+    tree = T.KDTree(sc["verts"], sc["normals"], sc["albedo"], sc["emission"], empty_leaves=True)
+    w, depth = 512, 6
+    conf = T.TracerConfig(width=w, max_recursion_depth=depth, num_pixel_samples=1, num_monte_carlo_samples=1,
+                          flags=TN_RENDER_COSINE | TN_RENDER_AREA_LIGHTS, seed=12)
+    out = {}
+    for rnd in range(2):
+        for defer in (False, True):
+            g = Guide.for_tree(tree, seed=2, epochs=5, mis_guided=0.7)
+            acc = torch.zeros((w, w, 4), dtype=torch.float32, device="cuda")
+            ms = []
+            for k in range(7):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                render_guided(tree, sc["camera"], sc["lights"], conf, g, acc,
+                              train_samples=(1 << 18) if k < 6 else 0, sample_offset=k, defer=defer)
+                torch.cuda.synchronize()
+                ms.append(round((time.perf_counter() - t0) * 1e3, 2))
+            g.close()
+            out[f"{'deferred' if defer else 'in_order'}_round{rnd}"] = {"ms_per_call": ms, "sum_ms": round(sum(ms), 2)}
+    note("guide pass probe done")
+    return {"workload": "occluded-light Cornell box + level-8 icosphere, 512x512, depth 6, cosine + area-light "
+                        "NEE, guide share 0.7; calls 0-5 record 2^18 samples each and train them (5 epochs, "
+                        "batch 16384), call 6 is an image pass (it trains the last set when deferred)",
+            "calls": out}
 
 
 def launch_probe():
