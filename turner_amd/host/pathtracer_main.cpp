@@ -63,7 +63,7 @@ MI355X options:
                                     it that renders by default (same hits, about 3x slower).
   --graph                           Replay the per-chunk launch sequence from a hipGraph.
   --guided                          Learned path guiding (needs -m 1): trains online, then renders.
-  --train-passes=<int>              Guiding: training passes of 2 spp [default: 16].
+  --train-passes=<int>              Guiding: training passes of 1 spp [default: 6].
   --pfm=<path>                      Also write the linear image (before exposure) as a float map.
   --print-config                    Print the parsed configuration and exit.
 )";
@@ -185,7 +185,7 @@ int main(int argc, char** argv) {
     TracerConfig conf;
     bool render_tree = false, reference_tree = false, guided = false;
     std::string pfm;
-    int train_passes = 16;
+    int train_passes = 6;
     try {
         Args a = parse(argc, argv);
         if (a.opt.count("--help") || a.pos.size() != 1) {
@@ -222,7 +222,7 @@ int main(int argc, char** argv) {
         reference_tree = a.opt.count("--reference-tree") > 0;
         pfm = get(a, "--pfm", "");
         guided = kRenderer == TN_RENDERER_PATHTRACER && a.opt.count("--guided") > 0;
-        train_passes = std::stoi(get(a, "--train-passes", "16"));
+        train_passes = std::stoi(get(a, "--train-passes", "6"));
         if (guided && (conf.num_monte_carlo_samples != 1 || train_passes < 0))
             throw std::runtime_error("--guided needs -m 1 and --train-passes >= 0");
         // config.h:29-33,119-126

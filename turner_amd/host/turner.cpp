@@ -384,9 +384,11 @@ RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vec
     if (hipMemset(img, 0, np * 16 * 2) != hipSuccess) throw Error(TN_ERR_HIP, "hipMemset failed");
     tn_render_params p = params_of(conf, conf.width, height);
     uint32_t offset = 0;
+    const uint32_t plain_flags = p.flags;
     auto run = [&](int spp, float* into, uint32_t train) {
         p.pixel_samples = spp;
         p.sample_offset = offset;
+        p.flags = plain_flags | ((train && sch.defer_training) ? TN_RENDER_DEFER_TRAINING : 0u);
         tn_render_stats st{};
         check(tn_render_guided(tree.handle(), &c, ls.data(), (uint32_t)ls.size(), &p, guide.handle(), train,
                                into, &st, nullptr));
@@ -397,6 +399,7 @@ RenderResult render_guided(const KDTree& tree, const Camera& cam, const std::vec
         res.stats.ms += st.ms;
     };
     for (int k = 0; k < sch.train_passes; ++k) run(sch.train_spp, scratch, sch.train_samples);
+    check(tn_guide_flush(guide.handle(), nullptr));  // a deferred last set trains before the image passes
     int left = conf.num_pixel_samples;
     while (left > 0) {
         const int n = std::min(left, std::max(1, sch.pass_spp));
